@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the SU(2) checkerboard sweep on one MI355X.
+
+Workload (the large single-lattice configuration): Lt = Lx = Ly = 128,
+periodic, heat-bath followed by one over-relaxation sweep, beta 5.0, hot
+start from seed 7, FP64 links.  One step = one heat-bath sweep (6 launches)
++ one over-relaxation sweep (6 launches) + the fused energy/plaquette
+measurement, with the reunitarisation every 100 steps, as in the
+reference's measurement loop (kernel.cu:192-208).  Warm-up steps play the
+role of thermalisation.
+
+Prints ONE JSON line.  value = link updates per second of the whole step
+(2 x 3 V updates per step).  roofline: the heat-bath sweep kernel, 240
+algorithmic bytes per link update (5 link reads + 1 link write of 32 B and
+48 B of RNG state read + written), timed per launch with HIP events on the
+stream it runs on.  cpu_baseline: the CPU checker (OpenMP C restatement of
+the reference) on the host cores, on a bounded number of steps of the same
+lattice.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_HB = 240.0  # per heat-bath link update, FP64 links
+BYTES_OR = 224.0  # per over-relaxation link update, FP64 links
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--L", type=int, default=128, help="Lt = Lx = Ly")
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64 links, 1 FP32 links")
+    ap.add_argument("--cpu-steps", type=int, default=-1, help="CPU baseline steps (-1: auto, 0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--roofline-sweeps", type=int, default=20)
+    return ap.parse_args()
+
+
+def gpu_bench(args):
+    import qcd_2d_gpu_amd as q
+    from qcd_2d_gpu_amd import _lib as L
+
+    lib = q.load_library()
+    n = q.device_count()
+    if n < 1:
+        raise SystemExit("bench.py: no HIP device visible")
+    stream = C.c_void_p()
+    L.check(lib.su2_stream_create(C.byref(stream)), "stream_create")
+    Lt = Lx = Ly = args.L
+    g = q.Lattice(Lt, Lx, Ly, 5.0, 0.1, 1, precision=args.precision, stream=stream)
+    V = g.V
+    g.setup_kernel(7)
+    g.rand_init2()
+    res = q.DeviceBuffer(8 * 2 * max(1, args.steps))
+    res.zero(stream)
+
+    def step(i, res_slot=None):
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
+        if i % 100 == 0:
+            g.normalize()
+        if res_slot is not None:
+            g.en_av_into(False, res.value + 16 * res_slot, res.value + 16 * res_slot + 8)
+
+    for i in range(1, args.warmup + 1):
+        step(i, 0)
+    g.synchronize()
+    res.zero(stream)
+    g.synchronize()
+
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    L.check(lib.su2_event_create(C.byref(ev0)), "event")
+    L.check(lib.su2_event_create(C.byref(ev1)), "event")
+    t0 = time.perf_counter()
+    L.check(lib.su2_event_record(ev0, stream), "record")
+    for i in range(1, args.steps + 1):
+        step(i, i - 1)
+    L.check(lib.su2_event_record(ev1, stream), "record")
+    ms = C.c_float()
+    L.check(lib.su2_event_elapsed_ms(ev0, ev1, C.byref(ms)), "elapsed")
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = float(ms.value)
+    meas = res.download(np.float64, 2 * args.steps, stream).reshape(args.steps, 2)
+
+    # roofline pass: each heat-bath (and over-relaxation) launch bracketed by
+    # events on the stream it is launched on
+    nl = 6 * args.roofline_sweeps
+    evs = []
+    for _ in range(2 * nl + 2):
+        e = C.c_void_p()
+        L.check(lib.su2_event_create(C.byref(e)), "event")
+        evs.append(e)
+    k = 0
+    hb_ms, or_ms = [], []
+    g.synchronize()
+    for s in range(args.roofline_sweeps):
+        for upd in (q.UPDATE_LINK, q.UPDATE_OVERRELAX):
+            for mu in range(3):
+                for par in range(2):
+                    if upd == q.UPDATE_LINK:
+                        a, b = evs[2 * k], evs[2 * k + 1]
+                        L.check(lib.su2_event_record(a, stream), "record")
+                        g.mc_update(0, q.ALGO_HEATBATH, upd, mu, par)
+                        L.check(lib.su2_event_record(b, stream), "record")
+                        k += 1
+                    else:
+                        g.mc_update(0, q.ALGO_HEATBATH, upd, mu, par)
+    g.synchronize()
+    for i in range(k):
+        L.check(lib.su2_event_elapsed_ms(evs[2 * i], evs[2 * i + 1], C.byref(ms)), "elapsed")
+        hb_ms.append(float(ms.value))
+    # over-relaxation launches timed the same way
+    k2 = 0
+    for s in range(args.roofline_sweeps):
+        for mu in range(3):
+            for par in range(2):
+                a, b = evs[2 * k2], evs[2 * k2 + 1]
+                L.check(lib.su2_event_record(a, stream), "record")
+                g.mc_update(0, q.ALGO_HEATBATH, q.UPDATE_OVERRELAX, mu, par)
+                L.check(lib.su2_event_record(b, stream), "record")
+                k2 += 1
+    g.synchronize()
+    for i in range(k2):
+        L.check(lib.su2_event_elapsed_ms(evs[2 * i], evs[2 * i + 1], C.byref(ms)), "elapsed")
+        or_ms.append(float(ms.value))
+    for e in evs + [ev0, ev1]:
+        lib.su2_event_destroy(e)
+    det_ok = float(np.mean(meas[:, 1]))
+    g.close()
+    res.free()
+    lib.su2_stream_destroy(stream)
+    return dict(total_ms=total_ms, wall=wall, V=V, hb_ms=float(np.mean(hb_ms)), or_ms=float(np.mean(or_ms)),
+                hb_ms_min=float(np.min(hb_ms)), E=float(np.mean(meas[:, 0])), P=det_ok)
+
+
+def cpu_baseline(args, V_gpu):
+    from oracle import oracle as O
+
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    L = args.L
+    o = O.Oracle(L, L, L, 5.0)
+    o.setup_kernel(7)
+    o.init(0)
+    # one warm-up step, then time a bounded number of steps
+    o.time_sweeps(0, 1, 1, 1)
+    steps = args.cpu_steps
+    if steps < 0:
+        t1 = o.time_sweeps(0, 1, 1, 1)
+        steps = max(1, min(30, int(15.0 / max(t1, 1e-6))))
+    if steps == 0:
+        return None
+    sec = o.time_sweeps(0, 1, 1, steps)
+    ups = 2 * 3 * o.V * steps / sec
+    return {"value": ups, "unit": "link updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps (heat-bath + over-relaxation sweep) of the same {L}^3 lattice, "
+                      f"{sec:.1f} s, OpenMP C restatement of the reference"}
+
+
+def load_traffic():
+    """HBM bytes per heat-bath launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    p = os.path.join(ROOT, "profiles", "hb_sweep_128cube_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the single-lattice sweep has no multi-GPU path here", file=sys.stderr)
+        sys.exit(2)
+    r = gpu_bench(args)
+    V = r["V"]
+    ms_step = r["total_ms"] / args.steps
+    ups = 2 * 3 * V / (ms_step * 1e-3)
+    per_launch_updates = V / 2
+    hb_gbs = BYTES_HB * per_launch_updates / (r["hb_ms"] * 1e-3) / 1e9
+    or_gbs = BYTES_OR * per_launch_updates / (r["or_ms"] * 1e-3) / 1e9
+    step_bytes = 3 * V * (BYTES_HB + BYTES_OR) + 96.0 * V
+    cpu = None
+    if args.cpu_steps != 0:
+        cpu = cpu_baseline(args, V)
+    prec = "f64" if args.precision == 0 else "f64 (FP32 link storage)"
+    out = {
+        "metric": "link updates/s (SU(2) 128^3 FP64 heat-bath + over-relaxation sweep)",
+        "value": ups,
+        "unit": "link updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": prec,
+        "data": "synthetic (hot start, MRG32k3a seed 7)",
+        "config": {"workload": f"SU(2) 2+1D Wilson action, Lt=Lx=Ly={args.L}, periodic, beta=5.0, "
+                               "heat-bath + 1 over-relaxation sweep per step + energy/plaquette measurement",
+                   "Lt": args.L, "Lx": args.L, "Ly": args.L, "beta": 5.0, "boundary": "periodic",
+                   "links": "FP64" if args.precision == 0 else "FP32", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": hb_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": hb_gbs / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "kernel": "heat-bath sweep launch (one direction x parity)",
+                     "bytes_per_update": BYTES_HB, "updates_per_launch": per_launch_updates,
+                     "avg_launch_us": r["hb_ms"] * 1e3},
+        "overrelax_launch": {"avg_launch_us": r["or_ms"] * 1e3, "achieved_gbs": or_gbs},
+        "step_algorithmic_gbs": step_bytes / (ms_step * 1e-3) / 1e9,
+        "heatbath_updates_per_s_kernel": per_launch_updates / (r["hb_ms"] * 1e-3),
+        "observables": {"energy": r["E"], "plaquette": r["P"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,252 @@
+/*
+ * su2mc.h - C ABI of libsu2mc, the MI355X (gfx950) SU(2) Yang-Mills Monte
+ * Carlo hot path: the Wilson-action checkerboard sweep with Kennedy-Pendleton
+ * heat-bath, Metropolis and over-relaxation, periodic / PCC / PMC boundaries
+ * in y, and the fused energy/plaquette reduction.
+ *
+ * Drop-in boundary.  fezquerro/QCD_2d_GPU has no FFI: kernel.cu:main calls
+ * the header functions of functions/ directly (kernel.cu:15-22).  Each entry
+ * point below replaces one of those calls; the reference function it
+ * replaces is named beside it.  Conventions:
+ *   - The caller allocates every device buffer (as kernel.cu:78,82,145,150
+ *     does) with the sizes given by su2_*_bytes and passes raw device
+ *     pointers.  su2_malloc/su2_free are provided for callers without their
+ *     own HIP allocator.
+ *   - All entry points take the FULL-lattice struct (the reference's `l`,
+ *     kernel.cu:58), also where the reference passes the half-lattice `l2`.
+ *   - Work is enqueued on `stream` (a hipStream_t; NULL = the default
+ *     stream) and is asynchronous unless stated otherwise; the reference
+ *     uses the legacy default stream with implicit ordering.
+ *   - Every entry point returns an int status: SU2_OK (0) or a negative
+ *     SU2_E* code; su2_last_error() gives a message for the calling thread.
+ *     (The reference returns void and checks nothing.)
+ *   - Device field layout: structure of arrays, link(p, mu, h) at index
+ *     (p * 3 + mu) * (V/2) + h, 4 doubles (SU2_PREC_F64) or 4 floats
+ *     (SU2_PREC_F32) per link.  su2_field_to_ref/su2_field_from_ref convert
+ *     to and from the reference's array-of-structures layout, which is also
+ *     its Config_ file format (functions/output.h:85).
+ *   - RNG state: six uint32 planes of V/2 entries, 24 B per half-site
+ *     (replaces curandStateMRG32k3a_t[V/2], kernel.cu:77-78).
+ */
+#ifndef SU2MC_H
+#define SU2MC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SU2MC_ABI_VERSION 1
+
+/* ---- status codes ---- */
+#define SU2_OK 0
+#define SU2_EINVAL (-1)      /* bad argument */
+#define SU2_EODD_EXTENT (-2) /* an extent is odd or < 2 (kernel.cu:61-64) */
+#define SU2_ENOMEM (-3)      /* device or host allocation failed */
+#define SU2_EHIP (-4)        /* a HIP runtime call failed */
+#define SU2_EIO (-5)         /* file I/O failed (the reference exit(1)s) */
+#define SU2_ENODEV (-6)      /* no usable gfx950 device */
+#define SU2_EBUDGET (-7)     /* request exceeds the memory budget */
+#define SU2_ECANCELLED (-8)  /* job was cancelled */
+
+/* ---- enumerations (values follow kernel.cu:11-13,28,31) ---- */
+#define SU2_ALGO_METROPOLIS 0 /* kernel.cu:11 METROPOLIS */
+#define SU2_ALGO_HEATBATH 1   /* kernel.cu:12 HEATBATH */
+#define SU2_UPDATE_LINK 0     /* mc_updates.h:178 update == 0: link_update */
+#define SU2_UPDATE_OVERRELAX 1 /* mc_updates.h:180 update == 1: over_relaxation */
+#define SU2_BC_PERIODIC 0     /* kernel.cu:28 Boundary 0 */
+#define SU2_BC_PCC 1          /* Boundary 1: perfect colour conductor */
+#define SU2_BC_PMC 2          /* Boundary 2: perfect magnetic conductor */
+#define SU2_START_HOT 0       /* kernel.cu:31 init 0: rand_init2 / rand_init2_pcc */
+#define SU2_START_COLD 1      /* init 1: id_init2 */
+#define SU2_START_HAAR 2      /* extension: Haar-random links */
+#define SU2_PREC_F64 0        /* links stored as 4 doubles */
+#define SU2_PREC_F32 1        /* links stored as 4 floats; all arithmetic FP64 */
+
+/*
+ * Lattice parameters.  Same members, types and order as the reference's
+ * struct lattice (functions/lattice_setup.h:2-14), holding the FULL lattice:
+ * y, x, t = Ly, Lx, Lt; V = Lt*Lx*Ly; d = 3; dg = 4; dt = 12; beta and
+ * epsilon are float as in the reference; updates = Metropolis hits per link.
+ */
+typedef struct su2_lattice {
+    int y;
+    int x;
+    int t;
+    int V;
+    int d;
+    int dg;
+    int dt;
+    float beta;
+    float epsilon;
+    int updates;
+} su2_lattice;
+
+/* Fills *l for an Lt x Lx x Ly lattice (kernel.cu:58).  Returns
+ * SU2_EODD_EXTENT when an extent is odd or below 2 (kernel.cu:61-64). */
+int su2_lattice_init(su2_lattice *l, int Lt, int Lx, int Ly, float beta, float epsilon, int updates);
+
+/* Buffer sizes in bytes. */
+size_t su2_field_bytes(const su2_lattice *l, int precision); /* kernel.cu:82  12 V reals */
+size_t su2_rng_bytes(const su2_lattice *l);                  /* kernel.cu:78  V/2 streams */
+size_t su2_accept_bytes(const su2_lattice *l);               /* kernel.cu:145 int[V] */
+size_t su2_en_av_work_bytes(const su2_lattice *l);           /* scratch of su2_en_av */
+
+/* ---- set-up and starts ---- */
+
+/* setup_kernel (functions/lattice_setup.h:15-21): stream h = subsequence h
+ * of MRG32k3a(seed), offset 0, for h < V/2. */
+int su2_setup_kernel(const su2_lattice *l, uint32_t *states, uint64_t seed, void *stream);
+
+/* id_init2 (functions/lattice_setup.h:38-51): every link the identity. */
+int su2_id_init2(const su2_lattice *l, void *phi, int precision, void *stream);
+
+/* rand_init2 (functions/lattice_setup.h:24-37): hot start, each link the
+ * normalised vector of four (u - 0.5) draws (functions/su2.h:29-34). */
+int su2_rand_init2(const su2_lattice *l, void *phi, int precision, uint32_t *states, void *stream);
+
+/* rand_init2_pcc (functions/lattice_setup.h:52-81): as rand_init2 but the x
+ * and t links at y = 0 are the identity and consume no draws. */
+int su2_rand_init2_pcc(const su2_lattice *l, void *phi, int precision, uint32_t *states, void *stream);
+
+/* Extension (no counterpart in the reference): Haar-random hot start; with
+ * pcc != 0 the x and t links at y = 0 are the identity. */
+int su2_haar_init(const su2_lattice *l, void *phi, int precision, uint32_t *states, int pcc, void *stream);
+
+/* ---- the sweep ---- */
+
+/* mc_up_per (functions/mc_updates.h:467-475): one sweep, six launches in
+ * the order (y,even) (y,odd) (x,even) (x,odd) (t,even) (t,odd).
+ * algo = SU2_ALGO_HEATBATH or SU2_ALGO_METROPOLIS (the reference's compile-
+ * time ALGO, kernel.cu:13); update = SU2_UPDATE_LINK or SU2_UPDATE_OVERRELAX
+ * (the reference's `int update`).  accept[site] += accepted hits. */
+int su2_mc_up_per(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int algo,
+                  int update, void *stream);
+
+/* mc_up_pcc (functions/mc_updates_pcc.h:216-224): y links periodic; x and t
+ * links at y = 0 not updated.  update = SU2_UPDATE_OVERRELAX is an extension
+ * (the reference has no PCC over-relaxation). */
+int su2_mc_up_pcc(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int algo,
+                  int update, void *stream);
+
+/* mc_up_pmc (functions/mc_updates_pmc.h:249-257): y links periodic; for x
+ * and t links the y-plaquettes between y = Ly-1 and y = 0 are dropped from
+ * the staple.  update = SU2_UPDATE_OVERRELAX is an extension. */
+int su2_mc_up_pmc(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int algo,
+                  int update, void *stream);
+
+/* One (direction, parity) launch of the above: mu 0 = y, 1 = x, 2 = t;
+ * parity 0 = even, 1 = odd (one MC_update_{even,odd}_{y,x,t}[_pcc|_pmc]
+ * kernel, functions/mc_updates.h:129-465). */
+int su2_mc_update(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int bc, int algo,
+                  int update, int mu, int parity, void *stream);
+
+/* normalize (functions/lattice_setup.h:202-207): reunitarise all 3V links. */
+int su2_normalize(const su2_lattice *l, void *phi, int precision, void *stream);
+
+/* en_av<PMC> (functions/observables.h:99-141): *res += mean over sites of
+ * -(1 - P_xt) (energy), *res2 += mean over sites and planes of 1/2 tr P
+ * (average plaquette); res, res2 are device pointers, as in the reference
+ * (end + i - 1, end2 + i - 1 at kernel.cu:208).  work is device scratch of
+ * su2_en_av_work_bytes(l).  The sum is deterministic (fixed order, no
+ * atomics), so repeated calls give identical bits. */
+int su2_en_av(const su2_lattice *l, const void *phi, int precision, int pmc, double *res, double *res2, void *work,
+              void *stream);
+
+/* cudaMemset(accept, 0, V * sizeof(int)) of kernel.cu:145,190, done with a
+ * fill kernel. */
+int su2_accept_clear(const su2_lattice *l, int *accept, void *stream);
+
+/* mean_lattice(int*) (functions/statistics.h:17-28): copies accept to the
+ * host (synchronising the stream) and returns the mean acceptance
+ * sum / V / n and its error; n = sweeps * 3 * hits (kernel.cu:225,246). */
+int su2_mean_lattice(const su2_lattice *l, const int *accept, long long n, double *mean, double *err, void *stream);
+
+/* Conversion to/from the reference array-of-structures field layout
+ * double[12 V] (functions/lattice_setup.h:27-32; the Config_ file format,
+ * functions/output.h:85).  Both pointers are device pointers. */
+int su2_field_to_ref(const su2_lattice *l, const void *phi, int precision, double *phi_ref, void *stream);
+int su2_field_from_ref(const su2_lattice *l, const double *phi_ref, void *phi, int precision, void *stream);
+
+/* ---- device plumbing ---- */
+int su2_device_count(int *n);
+int su2_set_device(int device);
+int su2_malloc(void **ptr, size_t bytes);
+int su2_free(void *ptr);
+int su2_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream); /* synchronous */
+int su2_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream); /* synchronous */
+int su2_memset_zero(void *dst, size_t bytes, void *stream);
+int su2_stream_create(void **stream);
+int su2_stream_destroy(void *stream);
+int su2_stream_synchronize(void *stream);
+int su2_event_create(void **event);
+int su2_event_destroy(void *event);
+int su2_event_record(void *event, void *stream);
+int su2_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronises on stop */
+const char *su2_strerror(int status);
+const char *su2_last_error(void);
+int su2_abi_version(void);
+
+/* Launch block size of the sweep kernels (threads per block, a multiple of
+ * 64, at most 256); 0 restores the built-in default. */
+int su2_set_sweep_block(int threads);
+
+/* ---- host driver (kernel.cu:49-268) ---- */
+
+/*
+ * Run configuration: the compile-time #defines of kernel.cu:10-46 as run-
+ * time fields (README.md:37-50).  Zero-initialise, then set what you need;
+ * su2_run_config_default gives the reference's own defaults where they are
+ * meaningful.
+ */
+typedef struct su2_run_config {
+    int Lt, Lx, Ly;     /* d_t, d_x, d_y */
+    float beta;         /* betal */
+    int boundary;       /* Boundary: SU2_BC_* */
+    int algo;           /* ALGO: SU2_ALGO_* */
+    int readconf;       /* 0 new start, 1 read Config_ then thermalise, 2 read without thermalisation */
+    int init;           /* SU2_START_* (init: 0 hot, 1 cold; 2 Haar extension) */
+    int sconf;          /* 0: save the configuration after thermalisation */
+    int save;           /* 0: write the result files */
+    int N;              /* measurements */
+    int N_term;         /* thermalisation sweeps */
+    int N_cor;          /* sweeps between measurements */
+    int Nsave;          /* times the results are saved during the run */
+    float epsilon;      /* epsilonl */
+    int upd;            /* upd: Metropolis hits per link */
+    int W;              /* autocorrelation lags written */
+    int or_sweeps;      /* extension: over-relaxation sweeps after each update sweep */
+    int precision;      /* SU2_PREC_* */
+    int normalize_every; /* reunitarisation cadence in sweeps (reference: 100) */
+    uint64_t seed;      /* replaces time(NULL) at kernel.cu:79 */
+    const char *outdir; /* directory of the output files (NULL = ".") */
+    const char *nsave;  /* run counter in result file names (reference "1") */
+    const char *nconfr; /* counter of the Config_ file read */
+    const char *nconfs; /* counter of the Config_ file written */
+    int verbose;        /* print progress like the reference */
+    int device;         /* HIP device ordinal */
+} su2_run_config;
+
+typedef struct su2_run_result {
+    double *energy;      /* caller buffer of N doubles, or NULL */
+    double *plaquette;   /* caller buffer of N doubles, or NULL */
+    double E_mean, E_err; /* mean_variance of the energy series (statistics.h:30-39) */
+    double P_mean, P_err; /* same for the plaquette */
+    double accept;       /* mean acceptance (kernel.cu:246-249, PCC-corrected) */
+    double det_mean;     /* mean link determinant after the run (print_det) */
+    double seconds_therm; /* wall time of the thermalisation loop */
+    double seconds_meas; /* wall time of the measurement loop */
+    long long link_updates; /* link updates performed (update + over-relaxation) */
+    int Nterms;          /* total thermalisation sweeps of the configuration */
+} su2_run_result;
+
+void su2_run_config_default(su2_run_config *cfg);
+int su2_run(const su2_run_config *cfg, su2_run_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SU2MC_H */

@@ -1,0 +1,256 @@
+/*
+ * su2_fmath.h - project-owned elementary functions for the SU(2) update
+ * bodies, identical bit for bit on the x86-64 host (gcc, clang) and on
+ * gfx950 (hipcc).
+ *
+ * The heat-bath and Metropolis updates of the reference evaluate logf,
+ * cospif, sincospif, sqrtf, expf, exp and rsqrt
+ * (functions/mc_updates.h:9,14,48-49,53,55-56,58; functions/su2.h:5,17).
+ * A one-ulp difference in any of them can flip a Kennedy-Pendleton accept
+ * test and shift every later draw of that site's random stream, so the
+ * device kernels and the CPU checker must evaluate them identically.  Vendor
+ * libraries (glibc libm, the device math library) do not promise that, so
+ * this header builds each function from IEEE-754 basic operations only
+ * (+ - * / and sqrt, all correctly rounded on both targets) with a fixed
+ * evaluation order.  Every file that includes it must be compiled with
+ * floating-point contraction disabled (-ffp-contract=off); the pragma below
+ * enforces that for clang/hipcc.
+ *
+ * Methods (all from the mathematics; no vendor library internals):
+ *  - log: x = 2^e * m, m in [sqrt(2)/2, sqrt(2)); log m = 2 atanh(f),
+ *    f = (m-1)/(m+1), |f| < 0.172, evaluated by its Taylor series through
+ *    f^19 (truncation < 3e-17 relative); e*ln2 with a two-part (Cody-Waite)
+ *    ln2 so that e*ln2_hi is exact.
+ *  - exp: x = k ln2 + r, |r| <= ln2/2 (Cody-Waite), exp r by its Taylor
+ *    series through r^13/13! (truncation < 5e-18), then scaled by 2^k with
+ *    exact exponent arithmetic (two steps for subnormal results).
+ *  - sinpi/cospi: exact reduction of the argument modulo 2 and into
+ *    [-1/4, 1/4] by the identities sin(pi(1/2-q)) = cos(pi q),
+ *    sin(pi(1-q)) = sin(pi q), then Taylor series of sin and cos of
+ *    t = pi*q, |t| <= pi/4, through t^17/17! and t^16/16! (truncation
+ *    < 2e-18).
+ *  - float versions evaluate the double function on the exactly converted
+ *    argument and round once to float, so they are correctly rounded except
+ *    when the exact result lies within ~1e-16 relative of a float rounding
+ *    boundary.
+ *  - sqrtf = (float)sqrt((double)x) is the correctly rounded float square
+ *    root (53 >= 2*24+2, so the double rounding is innocuous); rsqrt(x) is
+ *    1.0/sqrt(x), two correctly rounded operations.
+ *
+ * The header is valid C99 (for the CPU checker) and HIP C++.
+ */
+#ifndef SU2_FMATH_H
+#define SU2_FMATH_H
+
+#include <stdint.h>
+
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+
+#if defined(__HIP__) || defined(__HIPCC__)
+#define SU2_FM_FN __host__ __device__ static inline __attribute__((always_inline))
+#else
+#define SU2_FM_FN static inline
+#endif
+
+/* Cody-Waite split of ln 2: LN2_HI has 32 significant bits, so k*LN2_HI is
+ * exact for |k| < 2^21. */
+#define SU2_LN2_HI 0x1.62e42fee00000p-1
+#define SU2_LN2_LO 0x1.a39ef35793c76p-33
+#define SU2_INV_LN2 0x1.71547652b82fep+0
+#define SU2_PI 0x1.921fb54442d18p+1
+#define SU2_SQRT2 0x1.6a09e667f3bcdp+0
+
+SU2_FM_FN uint64_t su2_fm_bits(double x)
+{
+    union { double d; uint64_t u; } v;
+    v.d = x;
+    return v.u;
+}
+
+SU2_FM_FN double su2_fm_from_bits(uint64_t u)
+{
+    union { double d; uint64_t u; } v;
+    v.u = u;
+    return v.d;
+}
+
+/* 2^k for -1022 <= k <= 1023, exact. */
+SU2_FM_FN double su2_fm_pow2(int k)
+{
+    return su2_fm_from_bits((uint64_t)(k + 1023) << 52);
+}
+
+SU2_FM_FN double su2_sqrt_rn(double x) { return __builtin_sqrt(x); }
+
+/* rsqrt of the reference (functions/su2.h:5,17; mc_updates.h:58) as two
+ * correctly rounded operations. */
+SU2_FM_FN double su2_rsqrt(double x) { return 1.0 / __builtin_sqrt(x); }
+
+/* sqrtf (functions/mc_updates.h:56), correctly rounded. */
+SU2_FM_FN float su2_sqrtf(float x) { return (float)__builtin_sqrt((double)x); }
+
+SU2_FM_FN float su2_fmaxf(float a, float b)
+{
+    /* fmaxf semantics for the non-NaN operands that occur here */
+    return (a > b) ? a : ((b > a) ? b : (a == 0.0f ? (a + b) : a));
+}
+
+/* Natural logarithm for finite x > 0. */
+SU2_FM_FN double su2_log_d(double x)
+{
+    uint64_t u = su2_fm_bits(x);
+    int e = (int)((u >> 52) & 0x7ff);
+    if (e == 0) { /* subnormal: scale into the normal range */
+        u = su2_fm_bits(x * 0x1p54);
+        e = (int)((u >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    double m = su2_fm_from_bits((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    if (m > SU2_SQRT2) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    double f = (m - 1.0) / (m + 1.0);
+    double s = f * f;
+    double p = 1.0 / 19.0;
+    p = 1.0 / 17.0 + s * p;
+    p = 1.0 / 15.0 + s * p;
+    p = 1.0 / 13.0 + s * p;
+    p = 1.0 / 11.0 + s * p;
+    p = 1.0 / 9.0 + s * p;
+    p = 1.0 / 7.0 + s * p;
+    p = 1.0 / 5.0 + s * p;
+    p = 1.0 / 3.0 + s * p;
+    double logm = (2.0 * f) + (2.0 * f) * (s * p);
+    double de = (double)e;
+    return de * SU2_LN2_HI + (de * SU2_LN2_LO + logm);
+}
+
+/* logf of the reference (functions/mc_updates.h:49) for x > 0. */
+SU2_FM_FN float su2_logf(float x)
+{
+    if (x == 0.0f) return -__builtin_inff();
+    return (float)su2_log_d((double)x);
+}
+
+/* Exponential in double. */
+SU2_FM_FN double su2_exp_d(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return __builtin_inf();
+    if (x < -745.1332191019412) return 0.0;
+    double kd = __builtin_floor(x * SU2_INV_LN2 + 0.5);
+    int k = (int)kd;
+    double r = (x - kd * SU2_LN2_HI) - kd * SU2_LN2_LO;
+    double p = 1.0 / 6227020800.0;              /* 1/13! */
+    p = 1.0 / 479001600.0 + r * p;              /* 1/12! */
+    p = 1.0 / 39916800.0 + r * p;               /* 1/11! */
+    p = 1.0 / 3628800.0 + r * p;                /* 1/10! */
+    p = 1.0 / 362880.0 + r * p;                 /* 1/9! */
+    p = 1.0 / 40320.0 + r * p;                  /* 1/8! */
+    p = 1.0 / 5040.0 + r * p;                   /* 1/7! */
+    p = 1.0 / 720.0 + r * p;                    /* 1/6! */
+    p = 1.0 / 120.0 + r * p;                    /* 1/5! */
+    p = 1.0 / 24.0 + r * p;                     /* 1/4! */
+    p = 1.0 / 6.0 + r * p;                      /* 1/3! */
+    p = 0.5 + r * p;                            /* 1/2! */
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    if (k > 1023) return (p * 2.0) * su2_fm_pow2(k - 1);
+    if (k < -1021) return (p * su2_fm_pow2(k + 600)) * 0x1p-600;
+    return p * su2_fm_pow2(k);
+}
+
+/* expf of the reference (functions/mc_updates.h:9). */
+SU2_FM_FN float su2_expf(float x) { return (float)su2_exp_d((double)x); }
+
+/* exp of the reference (functions/mc_updates.h:14). */
+SU2_FM_FN double su2_exp(double x) { return su2_exp_d(x); }
+
+/* sin(pi b) for |b| <= 1/4: t = pi b, sin t = t - t z S(z), z = t^2,
+ * S(z) = 1/3! - z/5! + ... - z^7/17!. */
+SU2_FM_FN double su2_fm_sinpi_k(double b)
+{
+    double t = b * SU2_PI;
+    double z = t * t;
+    double p = -1.0 / 355687428096000.0;        /* -1/17! */
+    p = 1.0 / 1307674368000.0 + z * p;          /*  1/15! */
+    p = -1.0 / 6227020800.0 + z * p;            /* -1/13! */
+    p = 1.0 / 39916800.0 + z * p;               /*  1/11! */
+    p = -1.0 / 362880.0 + z * p;                /* -1/9!  */
+    p = 1.0 / 5040.0 + z * p;                   /*  1/7!  */
+    p = -1.0 / 120.0 + z * p;                   /* -1/5!  */
+    p = 1.0 / 6.0 + z * p;                      /*  1/3!  */
+    return t - (t * z) * p;
+}
+
+/* cos(pi b) for |b| <= 1/4: cos t = 1 - z C(z),
+ * C(z) = 1/2! - z/4! + ... - z^7/16!. */
+SU2_FM_FN double su2_fm_cospi_k(double b)
+{
+    double t = b * SU2_PI;
+    double z = t * t;
+    double p = -1.0 / 20922789888000.0;         /* -1/16! */
+    p = 1.0 / 87178291200.0 + z * p;            /*  1/14! */
+    p = -1.0 / 479001600.0 + z * p;             /* -1/12! */
+    p = 1.0 / 3628800.0 + z * p;                /*  1/10! */
+    p = -1.0 / 40320.0 + z * p;                 /* -1/8!  */
+    p = 1.0 / 720.0 + z * p;                    /*  1/6!  */
+    p = -1.0 / 24.0 + z * p;                    /* -1/4!  */
+    p = 0.5 + z * p;                            /*  1/2!  */
+    return 1.0 - z * p;
+}
+
+/* Exact reduction: r = a - 2 round(a/2) in [-1, 1), q = |r|. */
+SU2_FM_FN double su2_fm_reduce2(double a, int *neg)
+{
+    double r = a - 2.0 * __builtin_floor(a * 0.5 + 0.5);
+    *neg = r < 0.0;
+    return *neg ? -r : r;
+}
+
+/* cos(pi a) in double for finite a. */
+SU2_FM_FN double su2_cospi_d(double a)
+{
+    int neg;
+    double q = su2_fm_reduce2(a, &neg);
+    if (q <= 0.25) return su2_fm_cospi_k(q);
+    if (q <= 0.75) return su2_fm_sinpi_k(0.5 - q);
+    return -su2_fm_cospi_k(1.0 - q);
+}
+
+/* sin(pi a) and cos(pi a) in double for finite a. */
+SU2_FM_FN void su2_sincospi_d(double a, double *sp, double *cp)
+{
+    int neg;
+    double q = su2_fm_reduce2(a, &neg);
+    double s, c;
+    if (q <= 0.25) {
+        s = su2_fm_sinpi_k(q);
+        c = su2_fm_cospi_k(q);
+    } else if (q <= 0.75) {
+        s = su2_fm_cospi_k(0.5 - q);
+        c = su2_fm_sinpi_k(0.5 - q);
+    } else {
+        s = su2_fm_sinpi_k(1.0 - q);
+        c = -su2_fm_cospi_k(1.0 - q);
+    }
+    *sp = neg ? -s : s;
+    *cp = c;
+}
+
+/* cospif of the reference (functions/mc_updates.h:48). */
+SU2_FM_FN float su2_cospif(float x) { return (float)su2_cospi_d((double)x); }
+
+/* sincospif of the reference (functions/mc_updates.h:55). */
+SU2_FM_FN void su2_sincospif(float x, float *sp, float *cp)
+{
+    double s, c;
+    su2_sincospi_d((double)x, &s, &c);
+    *sp = (float)s;
+    *cp = (float)c;
+}
+
+#endif /* SU2_FMATH_H */

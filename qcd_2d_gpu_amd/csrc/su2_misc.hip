@@ -1,0 +1,368 @@
+/*
+ * su2_misc.hip - the non-sweep kernels of the path: RNG set-up, starts,
+ * reunitarisation, the fused energy/plaquette reduction and the conversion
+ * to and from the reference's on-disk field layout.
+ */
+#include "su2_internal.h"
+#include "su2_update.h"
+
+namespace {
+
+__constant__ uint32_t c_jump1[SU2_MRG_NJUMP * 9];
+__constant__ uint32_t c_jump2[SU2_MRG_NJUMP * 9];
+
+template <typename LinkT> struct io;
+template <> struct io<su2_l64> {
+    __device__ static inline su2_q load(const su2_l64 *p)
+    {
+        su2_l64 v = *p;
+        return su2_q_make(v.a0, v.a1, v.a2, v.a3);
+    }
+    __device__ static inline void store(su2_l64 *p, su2_q q)
+    {
+        su2_l64 v;
+        v.a0 = q.a0;
+        v.a1 = q.a1;
+        v.a2 = q.a2;
+        v.a3 = q.a3;
+        *p = v;
+    }
+};
+template <> struct io<su2_l32> {
+    __device__ static inline su2_q load(const su2_l32 *p)
+    {
+        su2_l32 v = *p;
+        return su2_q_make((double)v.a0, (double)v.a1, (double)v.a2, (double)v.a3);
+    }
+    __device__ static inline void store(su2_l32 *p, su2_q q)
+    {
+        su2_l32 v;
+        v.a0 = (float)q.a0;
+        v.a1 = (float)q.a1;
+        v.a2 = (float)q.a2;
+        v.a3 = (float)q.a3;
+        *p = v;
+    }
+};
+
+__device__ static inline uint32_t gtid(void) { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+/* setup_kernel (functions/lattice_setup.h:15-21): stream h starts at
+ * subsequence h of the seeded generator, offset 0. */
+__global__ __launch_bounds__(256) void k_setup_rng(uint32_t *__restrict__ rng, uint32_t Vh, uint32_t s0, uint32_t s1,
+                                                   uint32_t s2, uint32_t s3, uint32_t s4, uint32_t s5)
+{
+    const uint32_t h = gtid();
+    if (h >= Vh) return;
+    uint32_t s[6] = {s0, s1, s2, s3, s4, s5};
+    su2_mrg_jump(s, h, c_jump1, c_jump2);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) rng[(size_t)i * Vh + h] = s[i];
+}
+
+/* Starts.  Draw order per half-site h: parity 0 links y, x, t, then parity 1
+ * links y, x, t (functions/lattice_setup.h:24-37).  PCC variant
+ * (lattice_setup.h:52-81): the x and t links of the site at y = 0 are the
+ * identity and consume no draws. */
+template <int START, typename LinkT>
+__global__ __launch_bounds__(256) void k_init(LinkT *__restrict__ U, uint32_t *__restrict__ rng, su2_kgeom g)
+{
+    const uint32_t h = gtid();
+    if (h >= g.Vh) return;
+    const size_t Vh = g.Vh;
+    if (START == 1) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) io<LinkT>::store(U + k * Vh + h, su2_q_id());
+        return;
+    }
+    const uint32_t q = su2_div(h, g.div_lyh);
+    const uint32_t yh = h - q * g.Lyh;
+    const uint32_t t = su2_div(q, g.div_lx);
+    const uint32_t x = q - t * g.Lx;
+    const uint32_t par = (t + x) & 1u;
+    uint32_t s[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * Vh + h];
+    for (int p = 0; p < 2; ++p) {
+        const uint32_t y = 2u * yh + (par ^ (uint32_t)p);
+        for (int mu = 0; mu < 3; ++mu) {
+            su2_q l;
+            const bool pcc = (START == 2 || START == 4);
+            if (pcc && mu != SU2_DIR_Y && y == 0u)
+                l = su2_q_id();
+            else if (START == 3 || START == 4)
+                l = su2_haar_link(s);
+            else
+                l = su2_rand_link(s);
+            io<LinkT>::store(U + (size_t)(p * 3 + mu) * Vh + h, l);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) rng[(size_t)i * Vh + h] = s[i];
+}
+
+/* normalize (functions/lattice_setup.h:202-207): every link a <- a / |a|. */
+template <typename LinkT> __global__ __launch_bounds__(256) void k_normalize(LinkT *__restrict__ U, uint32_t nlinks)
+{
+    const uint32_t i = gtid();
+    if (i >= nlinks) return;
+    io<LinkT>::store(U + i, su2_q_normalize(io<LinkT>::load(U + i)));
+}
+
+/* The three plaquettes at site n of parity p (functions/observables.h:1-21),
+ * each returned as 1 - 1/2 tr P:
+ *   P_yx = U_y(n) U_x(n+y) U_y(n+x)^dag U_x(n)^dag
+ *   P_yt = U_y(n) U_t(n+y) U_y(n+t)^dag U_t(n)^dag
+ *   P_xt = U_x(n) U_t(n+x) U_x(n+t)^dag U_t(n)^dag
+ * The reduction of each site (functions/observables.h:99-141) gives
+ * E_site = -(1 - P_xt) and P_site = (3 - a - b - c) / 3; with PMC the y-x and
+ * y-t plaquettes of y = Ly-1 count as 1 (a = b = 0). */
+template <int PMC, typename LinkT>
+__device__ static inline void site_plaquettes(const LinkT *__restrict__ U, const su2_kgeom &g, uint32_t p, uint32_t h,
+                                              double *e, double *pl)
+{
+    const size_t Vh = g.Vh;
+    const uint32_t q = su2_div(h, g.div_lyh);
+    const uint32_t yh = h - q * g.Lyh;
+    const uint32_t t = su2_div(q, g.div_lx);
+    const uint32_t x = q - t * g.Lx;
+    const uint32_t y = 2u * yh + (((t + x) & 1u) ^ p);
+    const uint32_t yp = (y + 1u == g.Ly) ? 0u : y + 1u;
+    const uint32_t xp = (x + 1u == g.Lx) ? 0u : x + 1u;
+    const uint32_t tp = (t + 1u == g.Lt) ? 0u : t + 1u;
+    const uint32_t h_py = (yp >> 1) + x * g.Lyh + t * g.plane;
+    const uint32_t h_px = yh + xp * g.Lyh + t * g.plane;
+    const uint32_t h_pt = yh + x * g.Lyh + tp * g.plane;
+    const LinkT *Us = U + (size_t)(p * 3) * Vh;
+    const LinkT *Uo = U + (size_t)((1u - p) * 3) * Vh;
+
+    const su2_q uy = io<LinkT>::load(Us + 0 * Vh + h);
+    const su2_q ux = io<LinkT>::load(Us + 1 * Vh + h);
+    const su2_q ut = io<LinkT>::load(Us + 2 * Vh + h);
+    const su2_q ux_py = io<LinkT>::load(Uo + 1 * Vh + h_py);
+    const su2_q ut_py = io<LinkT>::load(Uo + 2 * Vh + h_py);
+    const su2_q uy_px = io<LinkT>::load(Uo + 0 * Vh + h_px);
+    const su2_q ut_px = io<LinkT>::load(Uo + 2 * Vh + h_px);
+    const su2_q uy_pt = io<LinkT>::load(Uo + 0 * Vh + h_pt);
+    const su2_q ux_pt = io<LinkT>::load(Uo + 1 * Vh + h_pt);
+
+    double a = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(uy, ux_py), uy_px), ux);
+    double b = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(uy, ut_py), uy_pt), ut);
+    double c = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(ux, ut_px), ux_pt), ut);
+    if (PMC && y == g.Ly - 1u) {
+        a = 0.0;
+        b = 0.0;
+    }
+    *e = -c;
+    *pl = (3.0 - a - b - c) / 3.0;
+}
+
+/* Deterministic block sum of two doubles: wavefront shuffles in a fixed
+ * butterfly, then the wavefront totals in order. */
+__device__ static inline void block_sum2(double &u, double &v)
+{
+    __shared__ double sh[2][16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        u += __shfl_xor(u, off, 64);
+        v += __shfl_xor(v, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) {
+        sh[0][wave] = u;
+        sh[1][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double su = 0.0, sv = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            su += sh[0][w];
+            sv += sh[1][w];
+        }
+        u = su;
+        v = sv;
+    }
+}
+
+/* Pass 1 of en_av: one thread per site (p, h), site index s = p * Vh + h as
+ * in the reference's grid over V; block partial sums to work[2 * block]. */
+template <int PMC, typename LinkT>
+__global__ __launch_bounds__(256) void k_en_av_partial(const LinkT *__restrict__ U, su2_kgeom g,
+                                                       double *__restrict__ work)
+{
+    const uint32_t s = gtid();
+    double e = 0.0, pl = 0.0;
+    if (s < 2u * g.Vh) {
+        const uint32_t p = s >= g.Vh ? 1u : 0u;
+        site_plaquettes<PMC, LinkT>(U, g, p, s - p * g.Vh, &e, &pl);
+    }
+    block_sum2(e, pl);
+    if (threadIdx.x == 0) {
+        work[2 * blockIdx.x] = e;
+        work[2 * blockIdx.x + 1] = pl;
+    }
+}
+
+/* Pass 2: one block adds the partials in a fixed order and accumulates
+ * sum / V into res, res2 (the reference adds into zeroed slots with
+ * atomicAdd, functions/observables.h:137-140). */
+__global__ __launch_bounds__(256) void k_en_av_final(const double *__restrict__ work, uint32_t nparts, double invV,
+                                                     double *__restrict__ res, double *__restrict__ res2)
+{
+    double e = 0.0, pl = 0.0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+        e += work[2 * i];
+        pl += work[2 * i + 1];
+    }
+    block_sum2(e, pl);
+    if (threadIdx.x == 0) {
+        *res += e * invV;
+        *res2 += pl * invV;
+    }
+}
+
+/* Reference layout (functions/lattice_setup.h:27-32, output.h:85):
+ * phi[((p * Vh + h) * 3 + mu) * 4 + c]. */
+template <typename LinkT>
+__global__ __launch_bounds__(256) void k_to_ref(const LinkT *__restrict__ U, double *__restrict__ phi, uint32_t Vh)
+{
+    const uint32_t i = gtid();
+    if (i >= 6u * Vh) return;
+    const uint32_t pm = i / Vh, h = i - pm * Vh, p = pm / 3u, mu = pm - 3u * p;
+    su2_q l = io<LinkT>::load(U + i);
+    double *d = phi + ((size_t)(p * Vh + h) * 3 + mu) * 4;
+    d[0] = l.a0;
+    d[1] = l.a1;
+    d[2] = l.a2;
+    d[3] = l.a3;
+}
+
+template <typename LinkT>
+__global__ __launch_bounds__(256) void k_from_ref(const double *__restrict__ phi, LinkT *__restrict__ U, uint32_t Vh)
+{
+    const uint32_t i = gtid();
+    if (i >= 6u * Vh) return;
+    const uint32_t pm = i / Vh, h = i - pm * Vh, p = pm / 3u, mu = pm - 3u * p;
+    const double *d = phi + ((size_t)(p * Vh + h) * 3 + mu) * 4;
+    io<LinkT>::store(U + i, su2_q_make(d[0], d[1], d[2], d[3]));
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int *__restrict__ p, int v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+bool g_jump_uploaded[64];
+
+hipError_t ensure_jump_tables(void)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev >= 0 && dev < 64 && g_jump_uploaded[dev]) return hipSuccess;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_jump1), su2_mrg_jump1_host, sizeof(su2_mrg_jump1_host));
+    if (e != hipSuccess) return e;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_jump2), su2_mrg_jump2_host, sizeof(su2_mrg_jump2_host));
+    if (e != hipSuccess) return e;
+    if (dev >= 0 && dev < 64) g_jump_uploaded[dev] = true;
+    return hipSuccess;
+}
+
+} // namespace
+
+hipError_t su2_launch_setup_rng(uint32_t *rng, const su2_kgeom &g, uint64_t seed, hipStream_t st)
+{
+    hipError_t e = ensure_jump_tables();
+    if (e != hipSuccess) return e;
+    uint32_t s[6];
+    su2_mrg_seed(seed, s);
+    const uint32_t nb = (g.Vh + 255) / 256;
+    hipLaunchKernelGGL(k_setup_rng, dim3(nb), dim3(256), 0, st, rng, g.Vh, s[0], s[1], s[2], s[3], s[4], s[5]);
+    return hipGetLastError();
+}
+
+template <typename LinkT>
+static hipError_t launch_init_t(int start, void *U, uint32_t *rng, const su2_kgeom &g, hipStream_t st)
+{
+    const uint32_t nb = (g.Vh + 255) / 256;
+    LinkT *u = (LinkT *)U;
+    switch (start) {
+    case 0: hipLaunchKernelGGL((k_init<0, LinkT>), dim3(nb), dim3(256), 0, st, u, rng, g); break;
+    case 1: hipLaunchKernelGGL((k_init<1, LinkT>), dim3(nb), dim3(256), 0, st, u, rng, g); break;
+    case 2: hipLaunchKernelGGL((k_init<2, LinkT>), dim3(nb), dim3(256), 0, st, u, rng, g); break;
+    case 3: hipLaunchKernelGGL((k_init<3, LinkT>), dim3(nb), dim3(256), 0, st, u, rng, g); break;
+    case 4: hipLaunchKernelGGL((k_init<4, LinkT>), dim3(nb), dim3(256), 0, st, u, rng, g); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t su2_launch_init(int start, int f32, void *U, uint32_t *rng, const su2_kgeom &g, hipStream_t st)
+{
+    return f32 ? launch_init_t<su2_l32>(start, U, rng, g, st) : launch_init_t<su2_l64>(start, U, rng, g, st);
+}
+
+hipError_t su2_launch_normalize(int f32, void *U, const su2_kgeom &g, hipStream_t st)
+{
+    const uint32_t n = 6u * g.Vh;
+    const uint32_t nb = (n + 255) / 256;
+    if (f32)
+        hipLaunchKernelGGL(k_normalize<su2_l32>, dim3(nb), dim3(256), 0, st, (su2_l32 *)U, n);
+    else
+        hipLaunchKernelGGL(k_normalize<su2_l64>, dim3(nb), dim3(256), 0, st, (su2_l64 *)U, n);
+    return hipGetLastError();
+}
+
+size_t su2_en_av_work_doubles(const su2_kgeom &g) { return 2 * (size_t)((2u * g.Vh + 255) / 256); }
+
+hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g, double *res, double *res2,
+                            double *work, hipStream_t st)
+{
+    const uint32_t nb = (2u * g.Vh + 255) / 256;
+    if (f32) {
+        if (pmc)
+            hipLaunchKernelGGL((k_en_av_partial<1, su2_l32>), dim3(nb), dim3(256), 0, st, (const su2_l32 *)U, g, work);
+        else
+            hipLaunchKernelGGL((k_en_av_partial<0, su2_l32>), dim3(nb), dim3(256), 0, st, (const su2_l32 *)U, g, work);
+    } else {
+        if (pmc)
+            hipLaunchKernelGGL((k_en_av_partial<1, su2_l64>), dim3(nb), dim3(256), 0, st, (const su2_l64 *)U, g, work);
+        else
+            hipLaunchKernelGGL((k_en_av_partial<0, su2_l64>), dim3(nb), dim3(256), 0, st, (const su2_l64 *)U, g, work);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const double invV = 1.0 / (2.0 * (double)g.Vh);
+    hipLaunchKernelGGL(k_en_av_final, dim3(1), dim3(256), 0, st, (const double *)work, nb, invV, res, res2);
+    return hipGetLastError();
+}
+
+hipError_t su2_launch_to_ref(int f32, const void *U, double *phi_ref, const su2_kgeom &g, hipStream_t st)
+{
+    const uint32_t nb = (6u * g.Vh + 255) / 256;
+    if (f32)
+        hipLaunchKernelGGL(k_to_ref<su2_l32>, dim3(nb), dim3(256), 0, st, (const su2_l32 *)U, phi_ref, g.Vh);
+    else
+        hipLaunchKernelGGL(k_to_ref<su2_l64>, dim3(nb), dim3(256), 0, st, (const su2_l64 *)U, phi_ref, g.Vh);
+    return hipGetLastError();
+}
+
+hipError_t su2_launch_from_ref(int f32, const double *phi_ref, void *U, const su2_kgeom &g, hipStream_t st)
+{
+    const uint32_t nb = (6u * g.Vh + 255) / 256;
+    if (f32)
+        hipLaunchKernelGGL(k_from_ref<su2_l32>, dim3(nb), dim3(256), 0, st, phi_ref, (su2_l32 *)U, g.Vh);
+    else
+        hipLaunchKernelGGL(k_from_ref<su2_l64>, dim3(nb), dim3(256), 0, st, phi_ref, (su2_l64 *)U, g.Vh);
+    return hipGetLastError();
+}
+
+hipError_t su2_launch_fill_i32(int *p, int v, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const size_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_fill_i32, dim3((uint32_t)nb), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}

@@ -1,0 +1,125 @@
+/*
+ * su2_quat.h - SU(2) link algebra in the quaternion ("sigma") basis.
+ *
+ * A link is four reals a0..a3 standing for the matrix
+ *     [[a0 + i a1,  a2 + i a3], [-a2 + i a3,  a0 - i a1]]
+ * (functions/su2.h:98-100).  The products below are the 2x2 matrix products
+ * in that basis.  Their term order follows functions/su2.h:50-76 exactly:
+ * with contraction disabled that fixes every rounding, which is what lets
+ * the device kernels agree bit for bit with the CPU checker.
+ */
+#ifndef SU2_QUAT_H
+#define SU2_QUAT_H
+
+#include "su2_fmath.h"
+
+#if defined(__HIP__) || defined(__HIPCC__)
+#define SU2_Q_FN __host__ __device__ static inline __attribute__((always_inline))
+#else
+#define SU2_Q_FN static inline
+#endif
+
+typedef struct su2_q {
+    double a0, a1, a2, a3;
+} su2_q;
+
+SU2_Q_FN su2_q su2_q_make(double a0, double a1, double a2, double a3)
+{
+    su2_q r;
+    r.a0 = a0;
+    r.a1 = a1;
+    r.a2 = a2;
+    r.a3 = a3;
+    return r;
+}
+
+SU2_Q_FN su2_q su2_q_zero(void) { return su2_q_make(0.0, 0.0, 0.0, 0.0); }
+SU2_Q_FN su2_q su2_q_id(void) { return su2_q_make(1.0, 0.0, 0.0, 0.0); }
+
+/* A B  (functions/su2.h:50-55) */
+SU2_Q_FN su2_q su2_q_mul(su2_q A, su2_q B)
+{
+    return su2_q_make(A.a0 * B.a0 - A.a1 * B.a1 - A.a2 * B.a2 - A.a3 * B.a3,
+                      A.a1 * B.a0 + A.a0 * B.a1 + A.a2 * B.a3 - A.a3 * B.a2,
+                      A.a2 * B.a0 + A.a3 * B.a1 + A.a0 * B.a2 - A.a1 * B.a3,
+                      A.a3 * B.a0 - A.a2 * B.a1 + A.a0 * B.a3 + A.a1 * B.a2);
+}
+
+/* A B^dagger  (functions/su2.h:56-61) */
+SU2_Q_FN su2_q su2_q_mul_nd(su2_q A, su2_q B)
+{
+    return su2_q_make(A.a0 * B.a0 + A.a1 * B.a1 + A.a2 * B.a2 + A.a3 * B.a3,
+                      A.a1 * B.a0 - A.a0 * B.a1 - A.a2 * B.a3 + A.a3 * B.a2,
+                      A.a2 * B.a0 - A.a3 * B.a1 - A.a0 * B.a2 + A.a1 * B.a3,
+                      A.a3 * B.a0 + A.a2 * B.a1 - A.a0 * B.a3 - A.a1 * B.a2);
+}
+
+/* A^dagger B  (functions/su2.h:62-67) */
+SU2_Q_FN su2_q su2_q_mul_dn(su2_q A, su2_q B)
+{
+    return su2_q_make(A.a0 * B.a0 + A.a1 * B.a1 + A.a2 * B.a2 + A.a3 * B.a3,
+                      -A.a1 * B.a0 + A.a0 * B.a1 - A.a2 * B.a3 + A.a3 * B.a2,
+                      -A.a2 * B.a0 - A.a3 * B.a1 + A.a0 * B.a2 + A.a1 * B.a3,
+                      -A.a3 * B.a0 + A.a2 * B.a1 + A.a0 * B.a3 - A.a1 * B.a2);
+}
+
+/* A^dagger B^dagger  (functions/su2.h:68-73) */
+SU2_Q_FN su2_q su2_q_mul_dd(su2_q A, su2_q B)
+{
+    return su2_q_make(A.a0 * B.a0 - A.a1 * B.a1 - A.a2 * B.a2 - A.a3 * B.a3,
+                      -A.a1 * B.a0 - A.a0 * B.a1 + A.a2 * B.a3 - A.a3 * B.a2,
+                      -A.a2 * B.a0 + A.a3 * B.a1 - A.a0 * B.a2 - A.a1 * B.a3,
+                      -A.a3 * B.a0 - A.a2 * B.a1 - A.a0 * B.a3 + A.a1 * B.a2);
+}
+
+/* First component of A B^dagger: 1/2 Re tr(A B^dagger). */
+SU2_Q_FN double su2_q_mul_nd_re(su2_q A, su2_q B)
+{
+    return A.a0 * B.a0 + A.a1 * B.a1 + A.a2 * B.a2 + A.a3 * B.a3;
+}
+
+/* 1/2 Re tr(A B)  (functions/su2.h:74-76) */
+SU2_Q_FN double su2_q_retr(su2_q A, su2_q B)
+{
+    return A.a0 * B.a0 - A.a1 * B.a1 - A.a2 * B.a2 - A.a3 * B.a3;
+}
+
+/* A + B, component-wise (functions/su2.h:77-80) */
+SU2_Q_FN su2_q su2_q_add(su2_q A, su2_q B)
+{
+    return su2_q_make(A.a0 + B.a0, A.a1 + B.a1, A.a2 + B.a2, A.a3 + B.a3);
+}
+
+/* A - B (functions/su2.h:81-84) */
+SU2_Q_FN su2_q su2_q_sub(su2_q A, su2_q B)
+{
+    return su2_q_make(A.a0 - B.a0, A.a1 - B.a1, A.a2 - B.a2, A.a3 - B.a3);
+}
+
+SU2_Q_FN su2_q su2_q_scale(su2_q A, double s)
+{
+    return su2_q_make(A.a0 * s, A.a1 * s, A.a2 * s, A.a3 * s);
+}
+
+/* |a|^2 = det  (functions/su2.h:1-3) */
+SU2_Q_FN double su2_q_det(su2_q A)
+{
+    return A.a0 * A.a0 + A.a1 * A.a1 + A.a2 * A.a2 + A.a3 * A.a3;
+}
+
+/* a <- a / |a|  (functions/su2.h:4-8) */
+SU2_Q_FN su2_q su2_q_normalize(su2_q A)
+{
+    return su2_q_scale(A, su2_rsqrt(su2_q_det(A)));
+}
+
+/* a <- a / |a|, returning |a| as det * (1/sqrt det)  (functions/su2.h:16-21) */
+SU2_Q_FN su2_q su2_q_normalize_det(su2_q A, double *norm)
+{
+    double det = su2_q_det(A);
+    double inv = su2_rsqrt(det);
+    *norm = det * inv;
+    return su2_q_scale(A, inv);
+}
+
+#endif /* SU2_QUAT_H */

@@ -1,0 +1,96 @@
+"""The C ABI library: loads without a GPU, exports every entry point that
+include/su2mc.h declares, and validates arguments on the host (no kernel is
+launched by these tests)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import qcd_2d_gpu_amd as q
+from qcd_2d_gpu_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "su2mc.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(su2_[a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declarations_found():
+    names = declared_functions()
+    assert "su2_mc_up_per" in names and "su2_en_av" in names and "su2_run" in names
+    assert len(names) >= 40
+
+
+def test_library_exports_every_declared_symbol():
+    lib = q.load_library()
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = set(re.findall(r" T (su2_[a-z_0-9]+)", out.stdout))
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+    for n in declared_functions():
+        assert hasattr(lib, n)
+    # the Python binding covers the whole header
+    assert set(declared_functions()) == set(L.PROTOTYPES)
+
+
+def test_struct_layout_matches_reference_struct():
+    # struct lattice (functions/lattice_setup.h:2-14): 7 ints, 2 floats, 1 int
+    assert C.sizeof(L.Su2Lattice) == 40
+    assert [f[0] for f in L.Su2Lattice._fields_] == ["y", "x", "t", "V", "d", "dg", "dt", "beta", "epsilon",
+                                                    "updates"]
+
+
+def test_lattice_init_and_sizes():
+    lib = q.load_library()
+    l = q.make_lattice(128, 128, 128, 5.0)
+    assert (l.t, l.x, l.y, l.V, l.d, l.dg, l.dt) == (128, 128, 128, 2097152, 3, 4, 12)
+    assert lib.su2_field_bytes(C.byref(l), 0) == 12 * 2097152 * 8
+    assert lib.su2_field_bytes(C.byref(l), 1) == 12 * 2097152 * 4
+    assert lib.su2_rng_bytes(C.byref(l)) == 24 * 1048576
+    assert lib.su2_accept_bytes(C.byref(l)) == 4 * 2097152
+    assert lib.su2_en_av_work_bytes(C.byref(l)) >= 16 * (2097152 // 256)
+
+
+@pytest.mark.parametrize("ext", [(15, 16, 16), (16, 15, 16), (16, 16, 3), (0, 4, 4), (2, 2, -2)])
+def test_odd_or_bad_extents_rejected(ext):
+    with pytest.raises(q.Su2Error) as e:
+        q.make_lattice(*ext, 3.0)
+    assert e.value.code == L.SU2_EODD_EXTENT
+
+
+def test_invalid_arguments_rejected_before_launch():
+    lib = q.load_library()
+    l = q.make_lattice(8, 8, 8, 3.0)
+    lp = C.byref(l)
+    fake = C.c_void_p(0x1000)
+    assert lib.su2_mc_up_per(lp, None, 0, fake, fake, 1, 0, None) == L.SU2_EINVAL
+    assert lib.su2_mc_up_per(lp, fake, 7, fake, fake, 1, 0, None) == L.SU2_EINVAL
+    assert lib.su2_mc_up_per(lp, fake, 0, fake, fake, 5, 0, None) == L.SU2_EINVAL
+    assert lib.su2_mc_up_per(lp, fake, 0, fake, None, 1, 0, None) == L.SU2_EINVAL
+    assert lib.su2_mc_update(lp, fake, 0, fake, fake, 0, 1, 0, 3, 0, None) == L.SU2_EINVAL
+    assert lib.su2_mc_update(lp, fake, 0, fake, fake, 3, 1, 0, 0, 0, None) == L.SU2_EINVAL
+    assert lib.su2_en_av(lp, fake, 0, 0, None, fake, fake, None) == L.SU2_EINVAL
+    bad = q.make_lattice(8, 8, 8, 3.0)
+    bad.V = 7
+    assert lib.su2_normalize(C.byref(bad), fake, 0, None) == L.SU2_EINVAL
+    assert "V = 7" in lib.su2_last_error().decode()
+    assert lib.su2_set_sweep_block(100) == L.SU2_EINVAL
+    assert lib.su2_set_sweep_block(0) == L.SU2_OK
+    assert lib.su2_strerror(L.SU2_EODD_EXTENT) == b"lattice extents must be even"
+
+
+def test_no_torch_in_product_package():
+    """The product talks to HIP through libsu2mc only."""
+    pkg = os.path.join(ROOT, "qcd_2d_gpu_amd")
+    for fn in os.listdir(pkg):
+        if fn.endswith(".py"):
+            src = open(os.path.join(pkg, fn)).read()
+            assert "import torch" not in src
+            assert "oracle" not in re.sub(r"#.*", "", src).lower().replace("oracle/", ""), fn
