@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=-1, help="CPU baseline steps (-1: auto, 0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--roofline-sweeps", type=int, default=20)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="after the run, launch the plain copy kernel 5 times on buffers of this size "
+                         "(reference dispatches for calibrating rocprofv3 byte counters)")
     return ap.parse_args()
 
 
@@ -139,6 +142,14 @@ def gpu_bench(args):
         or_ms.append(float(ms.value))
     for e in evs + [ev0, ev1]:
         lib.su2_event_destroy(e)
+    if args.copy_bytes:
+        src, dst = q.DeviceBuffer(args.copy_bytes), q.DeviceBuffer(args.copy_bytes)
+        src.zero(stream)
+        for _ in range(5):
+            L.check(lib.su2_device_copy(dst.ptr, src.ptr, args.copy_bytes, stream), "device_copy")
+        g.synchronize()
+        src.free()
+        dst.free()
     det_ok = float(np.mean(meas[:, 1]))
     g.close()
     res.free()
@@ -161,7 +172,7 @@ def cpu_baseline(args, V_gpu):
     steps = args.cpu_steps
     if steps < 0:
         t1 = o.time_sweeps(0, 1, 1, 1)
-        steps = max(1, min(30, int(15.0 / max(t1, 1e-6))))
+        steps = max(1, min(300, int(15.0 / max(t1, 1e-6))))
     if steps == 0:
         return None
     sec = o.time_sweeps(0, 1, 1, steps)
@@ -173,11 +184,14 @@ def cpu_baseline(args, V_gpu):
 
 def load_traffic():
     """HBM bytes per heat-bath launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
-    p = os.path.join(ROOT, "profiles", "hb_sweep_128cube_traffic.json")
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present:
+    the line "bytes_per_launch: N" of profiles/hb_sweep_128cube_traffic.txt."""
+    p = os.path.join(ROOT, "profiles", "hb_sweep_128cube_traffic.txt")
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
+            for line in f:
+                if line.startswith("bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
     return None
 
 

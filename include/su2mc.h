@@ -150,8 +150,9 @@ int su2_normalize(const su2_lattice *l, void *phi, int precision, void *stream);
  * -(1 - P_xt) (energy), *res2 += mean over sites and planes of 1/2 tr P
  * (average plaquette); res, res2 are device pointers, as in the reference
  * (end + i - 1, end2 + i - 1 at kernel.cu:208).  work is device scratch of
- * su2_en_av_work_bytes(l).  The sum is deterministic (fixed order, no
- * atomics), so repeated calls give identical bits. */
+ * su2_en_av_work_bytes(l) (block partial sums).  The sum is deterministic
+ * (partials added in a fixed order, no atomics), so repeated calls give
+ * identical bits.  Calls that share one work buffer must not overlap. */
 int su2_en_av(const su2_lattice *l, const void *phi, int precision, int pmc, double *res, double *res2, void *work,
               void *stream);
 
@@ -178,6 +179,9 @@ int su2_free(void *ptr);
 int su2_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream); /* synchronous */
 int su2_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream); /* synchronous */
 int su2_memset_zero(void *dst, size_t bytes, void *stream);
+/* Device-to-device copy by a plain streaming kernel (16 B per thread); also
+ * the reference kernel for calibrating hardware byte counters. */
+int su2_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 int su2_stream_create(void **stream);
 int su2_stream_destroy(void *stream);
 int su2_stream_synchronize(void *stream);

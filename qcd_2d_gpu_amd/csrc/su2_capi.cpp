@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -369,6 +370,15 @@ int su2_memset_zero(void *dst, size_t bytes, void *stream)
     if (!dst) return fail(SU2_EINVAL, "NULL");
     SU2_HIP(hipMemsetAsync(dst, 0, bytes, S(stream)));
     return SU2_OK;
+}
+
+int su2_device_copy(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes == 0) return SU2_OK;
+    if (!dst || !src || (bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+        return fail(SU2_EINVAL, "device_copy needs 16-byte aligned pointers and a size multiple of 16");
+    hipError_t e = su2_launch_copy(dst, src, bytes, S(stream));
+    return e == hipSuccess ? SU2_OK : hip_fail(e, "copy launch");
 }
 
 int su2_stream_create(void **stream)

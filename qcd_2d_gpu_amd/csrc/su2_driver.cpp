@@ -262,6 +262,7 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
     RUN_CHECK(su2_malloc(&end.p, (size_t)N * sizeof(double)));
     RUN_CHECK(su2_malloc(&end2.p, (size_t)N * sizeof(double)));
     RUN_CHECK(su2_malloc(&work.p, su2_en_av_work_bytes(&l)));
+    RUN_CHECK(su2_memset_zero(work.p, su2_en_av_work_bytes(&l), stream));
     RUN_CHECK(su2_malloc(&ref.p, (size_t)l.V * 12 * sizeof(double)));
     RUN_CHECK(su2_memset_zero(end.p, (size_t)N * sizeof(double), stream));
     RUN_CHECK(su2_memset_zero(end2.p, (size_t)N * sizeof(double), stream));

@@ -19,20 +19,22 @@
  * Methods (all from the mathematics; no vendor library internals):
  *  - log: x = 2^e * m, m in [sqrt(2)/2, sqrt(2)); log m = 2 atanh(f),
  *    f = (m-1)/(m+1), |f| < 0.172, evaluated by its Taylor series through
- *    f^19 (truncation < 3e-17 relative); e*ln2 with a two-part (Cody-Waite)
+ *    f^15 (truncation < 1.3e-12 relative); e*ln2 with a two-part (Cody-Waite)
  *    ln2 so that e*ln2_hi is exact.
  *  - exp: x = k ln2 + r, |r| <= ln2/2 (Cody-Waite), exp r by its Taylor
  *    series through r^13/13! (truncation < 5e-18), then scaled by 2^k with
  *    exact exponent arithmetic (two steps for subnormal results).
- *  - sinpi/cospi: exact reduction of the argument modulo 2 and into
- *    [-1/4, 1/4] by the identities sin(pi(1/2-q)) = cos(pi q),
- *    sin(pi(1-q)) = sin(pi q), then Taylor series of sin and cos of
- *    t = pi*q, |t| <= pi/4, through t^17/17! and t^16/16! (truncation
- *    < 2e-18).
+ *  - sinpi/cospi: exact reduction of the argument modulo 2 to r in [-1, 1),
+ *    q = |r|; then sin(pi q) = sin(pi min(q, 1-q)) and
+ *    cos(pi q) = sin(pi (1/2 - q)), so both come from one odd Taylor
+ *    polynomial of sin t on |t| <= pi/2 (through t^17/17!, truncation
+ *    < 5e-14) with no data-dependent branch: on the GPU every lane of a
+ *    wavefront runs the same instructions.
  *  - float versions evaluate the double function on the exactly converted
- *    argument and round once to float, so they are correctly rounded except
- *    when the exact result lies within ~1e-16 relative of a float rounding
- *    boundary.
+ *    argument and round once to float; with the truncations above they are
+ *    within one float ulp (correctly rounded unless the exact value lies
+ *    within ~1e-12 relative of a rounding boundary), the accuracy class of
+ *    the library functions the reference calls.
  *  - sqrtf = (float)sqrt((double)x) is the correctly rounded float square
  *    root (53 >= 2*24+2, so the double rounding is innocuous); rsqrt(x) is
  *    1.0/sqrt(x), two correctly rounded operations.
@@ -114,9 +116,7 @@ SU2_FM_FN double su2_log_d(double x)
     }
     double f = (m - 1.0) / (m + 1.0);
     double s = f * f;
-    double p = 1.0 / 19.0;
-    p = 1.0 / 17.0 + s * p;
-    p = 1.0 / 15.0 + s * p;
+    double p = 1.0 / 15.0;
     p = 1.0 / 13.0 + s * p;
     p = 1.0 / 11.0 + s * p;
     p = 1.0 / 9.0 + s * p;
@@ -169,41 +169,24 @@ SU2_FM_FN float su2_expf(float x) { return (float)su2_exp_d((double)x); }
 /* exp of the reference (functions/mc_updates.h:14). */
 SU2_FM_FN double su2_exp(double x) { return su2_exp_d(x); }
 
-/* sin(pi b) for |b| <= 1/4: t = pi b, sin t = t - t z S(z), z = t^2,
- * S(z) = 1/3! - z/5! + ... - z^7/17!. */
-SU2_FM_FN double su2_fm_sinpi_k(double b)
+/* sin(pi d) for |d| <= 1/2: t = pi d, sin t = t - t z S(z), z = t^2,
+ * S(z) = 1/3! - z/5! + ... + z^7/17!; truncation at t^19/19! < 5e-14. */
+SU2_FM_FN double su2_fm_sinpi_k(double d)
 {
-    double t = b * SU2_PI;
+    double t = d * SU2_PI;
     double z = t * t;
-    double p = -1.0 / 355687428096000.0;        /* -1/17! */
-    p = 1.0 / 1307674368000.0 + z * p;          /*  1/15! */
-    p = -1.0 / 6227020800.0 + z * p;            /* -1/13! */
-    p = 1.0 / 39916800.0 + z * p;               /*  1/11! */
-    p = -1.0 / 362880.0 + z * p;                /* -1/9!  */
-    p = 1.0 / 5040.0 + z * p;                   /*  1/7!  */
-    p = -1.0 / 120.0 + z * p;                   /* -1/5!  */
-    p = 1.0 / 6.0 + z * p;                      /*  1/3!  */
-    return t - (t * z) * p;
+    double p = 1.0 / 355687428096000.0;         /*  1/17! */
+    p = -1.0 / 1307674368000.0 + z * p;         /* -1/15! */
+    p = 1.0 / 6227020800.0 + z * p;             /*  1/13! */
+    p = -1.0 / 39916800.0 + z * p;              /* -1/11! */
+    p = 1.0 / 362880.0 + z * p;                 /*  1/9!  */
+    p = -1.0 / 5040.0 + z * p;                  /* -1/7!  */
+    p = 1.0 / 120.0 + z * p;                    /*  1/5!  */
+    p = -1.0 / 6.0 + z * p;                     /* -1/3!  */
+    return t + (t * z) * p;
 }
 
-/* cos(pi b) for |b| <= 1/4: cos t = 1 - z C(z),
- * C(z) = 1/2! - z/4! + ... - z^7/16!. */
-SU2_FM_FN double su2_fm_cospi_k(double b)
-{
-    double t = b * SU2_PI;
-    double z = t * t;
-    double p = -1.0 / 20922789888000.0;         /* -1/16! */
-    p = 1.0 / 87178291200.0 + z * p;            /*  1/14! */
-    p = -1.0 / 479001600.0 + z * p;             /* -1/12! */
-    p = 1.0 / 3628800.0 + z * p;                /*  1/10! */
-    p = -1.0 / 40320.0 + z * p;                 /* -1/8!  */
-    p = 1.0 / 720.0 + z * p;                    /*  1/6!  */
-    p = -1.0 / 24.0 + z * p;                    /* -1/4!  */
-    p = 0.5 + z * p;                            /*  1/2!  */
-    return 1.0 - z * p;
-}
-
-/* Exact reduction: r = a - 2 round(a/2) in [-1, 1), q = |r|. */
+/* Exact reduction: r = a - 2 round(a/2) in [-1, 1); returns q = |r|. */
 SU2_FM_FN double su2_fm_reduce2(double a, int *neg)
 {
     double r = a - 2.0 * __builtin_floor(a * 0.5 + 0.5);
@@ -211,14 +194,12 @@ SU2_FM_FN double su2_fm_reduce2(double a, int *neg)
     return *neg ? -r : r;
 }
 
-/* cos(pi a) in double for finite a. */
+/* cos(pi a) = sin(pi (1/2 - q)) in double for finite a. */
 SU2_FM_FN double su2_cospi_d(double a)
 {
     int neg;
     double q = su2_fm_reduce2(a, &neg);
-    if (q <= 0.25) return su2_fm_cospi_k(q);
-    if (q <= 0.75) return su2_fm_sinpi_k(0.5 - q);
-    return -su2_fm_cospi_k(1.0 - q);
+    return su2_fm_sinpi_k(0.5 - q);
 }
 
 /* sin(pi a) and cos(pi a) in double for finite a. */
@@ -226,19 +207,9 @@ SU2_FM_FN void su2_sincospi_d(double a, double *sp, double *cp)
 {
     int neg;
     double q = su2_fm_reduce2(a, &neg);
-    double s, c;
-    if (q <= 0.25) {
-        s = su2_fm_sinpi_k(q);
-        c = su2_fm_cospi_k(q);
-    } else if (q <= 0.75) {
-        s = su2_fm_cospi_k(0.5 - q);
-        c = su2_fm_sinpi_k(0.5 - q);
-    } else {
-        s = su2_fm_sinpi_k(1.0 - q);
-        c = -su2_fm_cospi_k(1.0 - q);
-    }
+    double s = su2_fm_sinpi_k(q < 1.0 - q ? q : 1.0 - q);
     *sp = neg ? -s : s;
-    *cp = c;
+    *cp = su2_fm_sinpi_k(0.5 - q);
 }
 
 /* cospif of the reference (functions/mc_updates.h:48). */

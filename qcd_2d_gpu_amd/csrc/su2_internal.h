@@ -126,5 +126,6 @@ size_t su2_en_av_work_doubles(const su2_kgeom &g);
 hipError_t su2_launch_to_ref(int f32, const void *U, double *phi_ref, const su2_kgeom &g, hipStream_t st);
 hipError_t su2_launch_from_ref(int f32, const double *phi_ref, void *U, const su2_kgeom &g, hipStream_t st);
 hipError_t su2_launch_fill_i32(int *p, int value, size_t n, hipStream_t st);
+hipError_t su2_launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 
 #endif /* SU2_INTERNAL_H */

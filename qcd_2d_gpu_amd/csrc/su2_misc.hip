@@ -116,16 +116,13 @@ template <typename LinkT> __global__ __launch_bounds__(256) void k_normalize(Lin
  *   P_xt = U_x(n) U_t(n+x) U_x(n+t)^dag U_t(n)^dag
  * The reduction of each site (functions/observables.h:99-141) gives
  * E_site = -(1 - P_xt) and P_site = (3 - a - b - c) / 3; with PMC the y-x and
- * y-t plaquettes of y = Ly-1 count as 1 (a = b = 0). */
+ * y-t plaquettes of y = Ly-1 count as 1 (a = b = 0).  (t, x, y') is the
+ * decoded half-site shared by both parities. */
 template <int PMC, typename LinkT>
 __device__ static inline void site_plaquettes(const LinkT *__restrict__ U, const su2_kgeom &g, uint32_t p, uint32_t h,
-                                              double *e, double *pl)
+                                              uint32_t t, uint32_t x, uint32_t yh, double *e, double *pl)
 {
     const size_t Vh = g.Vh;
-    const uint32_t q = su2_div(h, g.div_lyh);
-    const uint32_t yh = h - q * g.Lyh;
-    const uint32_t t = su2_div(q, g.div_lx);
-    const uint32_t x = q - t * g.Lx;
     const uint32_t y = 2u * yh + (((t + x) & 1u) ^ p);
     const uint32_t yp = (y + 1u == g.Ly) ? 0u : y + 1u;
     const uint32_t xp = (x + 1u == g.Lx) ? 0u : x + 1u;
@@ -158,7 +155,7 @@ __device__ static inline void site_plaquettes(const LinkT *__restrict__ U, const
 }
 
 /* Deterministic block sum of two doubles: wavefront shuffles in a fixed
- * butterfly, then the wavefront totals in order. */
+ * butterfly, then the wavefront totals in order (result in thread 0). */
 __device__ static inline void block_sum2(double &u, double &v)
 {
     __shared__ double sh[2][16];
@@ -185,40 +182,57 @@ __device__ static inline void block_sum2(double &u, double &v)
     }
 }
 
-/* Pass 1 of en_av: one thread per site (p, h), site index s = p * Vh + h as
- * in the reference's grid over V; block partial sums to work[2 * block]. */
+/* Pass 1 of en_av, one site per thread.  Logical block b covers the sites
+ * (p = b & 1, h in chunk b >> 1): the two parities of a chunk of 256
+ * half-sites are adjacent blocks, which read each other's neighbourhoods,
+ * and the XCD-aware remap keeps runs of chunks on one XCD's L2.  Block sums
+ * go to work[2 b]. */
 template <int PMC, typename LinkT>
 __global__ __launch_bounds__(256) void k_en_av_partial(const LinkT *__restrict__ U, su2_kgeom g,
                                                        double *__restrict__ work)
 {
-    const uint32_t s = gtid();
+    const uint32_t b = su2_xcd_block();
+    const uint32_t p = b & 1u;
+    const uint32_t h = (b >> 1) * blockDim.x + threadIdx.x;
     double e = 0.0, pl = 0.0;
-    if (s < 2u * g.Vh) {
-        const uint32_t p = s >= g.Vh ? 1u : 0u;
-        site_plaquettes<PMC, LinkT>(U, g, p, s - p * g.Vh, &e, &pl);
+    if (h < g.Vh) {
+        const uint32_t q = su2_div(h, g.div_lyh);
+        const uint32_t yh = h - q * g.Lyh;
+        const uint32_t t = su2_div(q, g.div_lx);
+        const uint32_t x = q - t * g.Lx;
+        site_plaquettes<PMC, LinkT>(U, g, p, h, t, x, yh, &e, &pl);
     }
     block_sum2(e, pl);
     if (threadIdx.x == 0) {
-        work[2 * blockIdx.x] = e;
-        work[2 * blockIdx.x + 1] = pl;
+        work[2 * b] = e;
+        work[2 * b + 1] = pl;
     }
 }
 
-/* Pass 2: one block adds the partials in a fixed order and accumulates
- * sum / V into res, res2 (the reference adds into zeroed slots with
- * atomicAdd, functions/observables.h:137-140). */
-__global__ __launch_bounds__(256) void k_en_av_final(const double *__restrict__ work, uint32_t nparts, double invV,
-                                                     double *__restrict__ res, double *__restrict__ res2)
+/* Pass 2: one block of 1024 threads adds the partials in a fixed order (four
+ * accumulators per thread over a stride of 4 * blockDim, then the block
+ * sum) and accumulates sum / V into res, res2 (the reference adds into
+ * zeroed slots with atomicAdd, functions/observables.h:137-140). */
+__global__ __launch_bounds__(1024) void k_en_av_final(const double *__restrict__ work, uint32_t n, double invV,
+                                                      double *__restrict__ res, double *__restrict__ res2)
 {
-    double e = 0.0, pl = 0.0;
-    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
-        e += work[2 * i];
-        pl += work[2 * i + 1];
+    double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = threadIdx.x; i < n; i += 4 * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t j = i + k * blockDim.x;
+            if (j < n) {
+                se[k] += work[2 * j];
+                sp[k] += work[2 * j + 1];
+            }
+        }
     }
-    block_sum2(e, pl);
+    double te = (se[0] + se[1]) + (se[2] + se[3]);
+    double tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+    block_sum2(te, tp);
     if (threadIdx.x == 0) {
-        *res += e * invV;
-        *res2 += pl * invV;
+        *res += te * invV;
+        *res2 += tp * invV;
     }
 }
 
@@ -252,6 +266,13 @@ __global__ __launch_bounds__(256) void k_fill_i32(int *__restrict__ p, int v, si
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+/* Plain streaming copy, 16 B per thread per iteration. */
+__global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 bool g_jump_uploaded[64];
@@ -315,12 +336,17 @@ hipError_t su2_launch_normalize(int f32, void *U, const su2_kgeom &g, hipStream_
     return hipGetLastError();
 }
 
-size_t su2_en_av_work_doubles(const su2_kgeom &g) { return 2 * (size_t)((2u * g.Vh + 255) / 256); }
+/* en_av launch shape: one site per thread, two blocks (the two parities)
+ * per chunk of 256 half-sites. */
+static uint32_t en_av_blocks(const su2_kgeom &g) { return su2_grid_blocks(2ull * ((g.Vh + 255) / 256) * 256, 256); }
+
+/* partial sums, 2 per block */
+size_t su2_en_av_work_doubles(const su2_kgeom &g) { return 2 * (size_t)en_av_blocks(g); }
 
 hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g, double *res, double *res2,
                             double *work, hipStream_t st)
 {
-    const uint32_t nb = (2u * g.Vh + 255) / 256;
+    const uint32_t nb = en_av_blocks(g);
     if (f32) {
         if (pmc)
             hipLaunchKernelGGL((k_en_av_partial<1, su2_l32>), dim3(nb), dim3(256), 0, st, (const su2_l32 *)U, g, work);
@@ -335,7 +361,7 @@ hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const double invV = 1.0 / (2.0 * (double)g.Vh);
-    hipLaunchKernelGGL(k_en_av_final, dim3(1), dim3(256), 0, st, (const double *)work, nb, invV, res, res2);
+    hipLaunchKernelGGL(k_en_av_final, dim3(1), dim3(1024), 0, st, (const double *)work, nb, invV, res, res2);
     return hipGetLastError();
 }
 
@@ -364,5 +390,15 @@ hipError_t su2_launch_fill_i32(int *p, int v, size_t n, hipStream_t st)
     if (n == 0) return hipSuccess;
     const size_t nb = (n + 255) / 256;
     hipLaunchKernelGGL(k_fill_i32, dim3((uint32_t)nb), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t su2_launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    size_t nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_copy16, dim3((uint32_t)nb), dim3(256), 0, st, (uint4 *)dst, (const uint4 *)src, n);
     return hipGetLastError();
 }

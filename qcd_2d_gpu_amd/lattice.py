@@ -83,6 +83,7 @@ class Lattice:
         self.states = DeviceBuffer(self.lib.su2_rng_bytes(lp))
         self.accept_buf = DeviceBuffer(self.lib.su2_accept_bytes(lp))
         self.work = DeviceBuffer(self.lib.su2_en_av_work_bytes(lp))
+        self.work.zero(stream)
         self.res = DeviceBuffer(2 * 8)
         self.accept_clear()
 
