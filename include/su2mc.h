@@ -197,6 +197,106 @@ int su2_abi_version(void);
  * 64, at most 256); 0 restores the built-in default. */
 int su2_set_sweep_block(int threads);
 
+/* ---- batched lattices: one launch per (direction, parity) for many ----
+ *
+ * Extension (no counterpart in the reference, whose driver runs one lattice
+ * per process): an engine holds many lattices of different shapes,
+ * couplings, boundary conditions and seeds in one device arena and advances
+ * all of them with one persistent launch per (direction, parity).  Each job
+ * follows the reference's run loop (kernel.cu:161-208) on its own schedule
+ * and gives bit for bit the results it gives alone.  A tick advances every
+ * resident, unfinished job by one update sweep (+ its over-relaxation
+ * sweeps), with its reunitarisation and measurement when due.  Ticks can be
+ * replayed from one captured hipGraph (use_graph); jobs may join or leave
+ * between ticks.
+ */
+typedef struct su2_job {
+    int Lt, Lx, Ly;
+    float beta, epsilon;
+    int hits;            /* Metropolis hits per link */
+    int bc;              /* SU2_BC_* */
+    int algo;            /* SU2_ALGO_* */
+    int start;           /* SU2_START_* */
+    uint64_t seed;
+    int n_therm;         /* thermalisation sweeps (>= 1) */
+    int n_meas;          /* measurements */
+    int n_cor;           /* sweeps per measurement (>= 1) */
+    int or_sweeps;       /* over-relaxation sweeps after each update sweep (<= engine max_or) */
+    int normalize_every; /* reunitarisation cadence (0: 100, the reference's) */
+} su2_job;
+
+typedef struct su2_engine_config {
+    int precision;       /* SU2_PREC_* for every job of the engine */
+    int device;
+    int max_jobs;        /* resident jobs (0: 256) */
+    int max_or;          /* over-relaxation passes per tick (0: 1) */
+    size_t budget_bytes; /* device memory for job data (0: 1 GiB) */
+    int use_graph;       /* replay ticks from a captured hipGraph */
+    int block;           /* sweep workgroup size: 64, 128 or 256 (0: autotuned) */
+    int grid;            /* persistent workgroups per launch (0: autotuned) */
+} su2_engine_config;
+
+typedef struct su2_engine su2_engine;
+
+/* Device bytes a job occupies in the arena. */
+size_t su2_job_bytes(const su2_job *job, int precision);
+int su2_engine_create(const su2_engine_config *cfg, su2_engine **out);
+int su2_engine_destroy(su2_engine *e);
+/* Admits a job (allocates, seeds setup_kernel-style, starts).  Returns
+ * SU2_EBUDGET when its data does not fit in the free arena now and
+ * SU2_EINVAL for a bad job; *slot identifies it afterwards. */
+int su2_engine_add(su2_engine *e, const su2_job *job, int *slot);
+/* Enqueues n ticks (asynchronous). */
+int su2_engine_tick(su2_engine *e, int n);
+/* Ticks until every resident job is finished; synchronises. */
+int su2_engine_run(su2_engine *e);
+int su2_engine_sync(su2_engine *e);
+/* sweeps done and total sweeps of a job (host counters) */
+int su2_engine_job_status(su2_engine *e, int slot, int *done_sweeps, int *total_sweeps);
+/* Copies a job's results (synchronising): energies and plaquettes of its
+ * n_meas measurements, mean acceptance (kernel.cu:246-249 normalisation,
+ * PCC-corrected), and optionally its field in the reference layout
+ * (double[12 V]) and RNG state (uint32[6 * V/2]).  NULL skips an output. */
+int su2_engine_read(su2_engine *e, int slot, double *E, double *P, double *accept, double *field_ref,
+                    uint32_t *rng);
+/* Frees a job's slot and memory. */
+int su2_engine_remove(su2_engine *e, int slot);
+/* Times the candidate (block, grid) shapes on the resident jobs and keeps
+ * the fastest; the choice is cached per shape class for the process. */
+int su2_engine_autotune(su2_engine *e, int *block, int *grid);
+/* Device memory in use / free in the arena. */
+int su2_engine_memory(su2_engine *e, size_t *used, size_t *capacity);
+
+/* ---- serving: continuous batching of run requests ----
+ *
+ * Extension.  A server thread owns an engine: submitted jobs queue, are
+ * admitted as soon as a slot and arena memory are free (first that fits,
+ * in arrival order), run in the shared ticks with everything else resident,
+ * and are retired with their results when done.  Times are seconds on the
+ * server's steady clock.
+ */
+typedef struct su2_server su2_server;
+
+typedef struct su2_job_result {
+    long long id;
+    int status;          /* SU2_OK or an error code */
+    double t_submit, t_admit, t_done;
+    double E_mean, E_err, P_mean, P_err, accept;
+    long long link_updates;
+} su2_job_result;
+
+int su2_server_create(const su2_engine_config *cfg, su2_server **out);
+/* Stops the server thread after the jobs already submitted are done. */
+int su2_server_destroy(su2_server *s);
+int su2_server_submit(su2_server *s, const su2_job *job, long long *id);
+/* Moves up to max finished results into out; *n = number returned. */
+int su2_server_poll(su2_server *s, su2_job_result *out, int max, int *n);
+/* Blocks until no job is queued or running. */
+int su2_server_drain(su2_server *s);
+/* Series of a finished job (before it is polled): NULL skips. */
+int su2_server_series(su2_server *s, long long id, double *E, double *P, int n);
+double su2_server_now(su2_server *s);
+
 /* ---- host driver (kernel.cu:49-268) ---- */
 
 /*

@@ -10,12 +10,13 @@ from ._lib import (ALGO_HEATBATH, ALGO_METROPOLIS, BC_PCC, BC_PERIODIC, BC_PMC, 
                    START_HAAR, START_HOT, UPDATE_LINK, UPDATE_OVERRELAX, Su2Error)
 from .lattice import DeviceBuffer, Lattice, make_lattice
 from .run import run
+from .engine import Engine, Job, Server
 from . import stats
 
 __all__ = [
     "ALGO_HEATBATH", "ALGO_METROPOLIS", "BC_PCC", "BC_PERIODIC", "BC_PMC", "PREC_F32", "PREC_F64", "START_COLD",
     "START_HAAR", "START_HOT", "UPDATE_LINK", "UPDATE_OVERRELAX", "Su2Error", "DeviceBuffer", "Lattice",
-    "make_lattice", "run", "stats", "load_library", "device_count",
+    "make_lattice", "run", "stats", "load_library", "device_count", "Engine", "Job", "Server",
 ]
 
 

@@ -1,0 +1,45 @@
+/*
+ * su2_batch.h - device work list of the batched launches (su2_batch.hip)
+ * and their host launchers.
+ */
+#ifndef SU2_BATCH_H
+#define SU2_BATCH_H
+
+#include "su2_internal.h"
+
+/* One resident lattice.  All pointers are device pointers into the engine's
+ * arena. */
+struct su2_bdesc {
+    void *U;        /* field, SoA layout of su2_internal.h */
+    uint32_t *rng;  /* 6 planes of Vh */
+    int *accept;    /* 2 Vh */
+    double *part;   /* en_av partials, 2 per logical block */
+    double *resE;   /* n_meas energies */
+    double *resP;   /* n_meas plaquettes */
+    su2_kgeom g;
+    su2_kphys ph;
+    int bc, algo, slot;
+    int n_therm, n_cor, n_meas, nevery, or_sweeps;
+    uint32_t sw_begin; /* first virtual block in the sweep launches */
+    uint32_t en_begin; /* first virtual block in the en_av launch */
+    uint32_t en_nb;    /* en_av logical blocks of this lattice */
+    uint32_t nm_begin; /* first virtual block in the normalise launch */
+};
+
+struct su2_bhead {
+    uint32_t n_active;  /* descriptors in use */
+    uint32_t sw_blocks; /* total virtual blocks of a sweep launch */
+    uint32_t en_blocks;
+    uint32_t nm_blocks;
+};
+
+hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, const su2_bhead *H, const su2_bdesc *D, int parity,
+                             int or_pass, uint32_t grid, uint32_t block, hipStream_t st);
+hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *ctr, int post,
+                                 uint32_t grid, hipStream_t st);
+hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *ctr, uint32_t grid,
+                             uint32_t max_desc, hipStream_t st);
+hipError_t su2_launch_btick_end(const su2_bhead *H, const su2_bdesc *D, uint32_t *ctr, hipStream_t st);
+uint32_t su2_en_av_logical_blocks(const su2_kgeom &g);
+
+#endif /* SU2_BATCH_H */

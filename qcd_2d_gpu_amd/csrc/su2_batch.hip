@@ -1,0 +1,278 @@
+/*
+ * su2_batch.hip - batched launches over many lattices of different shapes,
+ * couplings and boundary conditions: one persistent launch per
+ * (direction, parity) for all of them, plus batched reunitarisation and
+ * energy/plaquette reduction.
+ *
+ * The work list lives in device memory (su2_bhead + a compact array of
+ * su2_bdesc, one per resident lattice).  Every lattice is cut into virtual
+ * blocks; block ranges are consecutive in the descriptor order, and a
+ * launch's workgroups walk the virtual blocks with a grid stride, finding
+ * their lattice by binary search on the range starts.  A virtual block
+ * never spans two lattices, so the boundary condition and algorithm are
+ * uniform in a workgroup.  Because the launch geometry does not depend on
+ * the work list, one captured hipGraph stays valid when lattices join or
+ * leave: only the descriptor table is rewritten between graph launches.
+ *
+ * Each lattice advances by its own schedule, the reference's run loop
+ * (kernel.cu:161-208): with s = sweeps done before this tick,
+ *   thermalisation  s < n_therm: normalise first if (s+1) % nevery == 0;
+ *                   after the last one (s = n_therm-1) normalise once more;
+ *   measurement     k = s - n_therm; after the last of n_cor sweeps of
+ *                   measurement i = k / n_cor + 1: normalise if
+ *                   i % nevery == 0, then en_av into slot i-1.
+ * The per-site code is the single-lattice code (su2_sweep_site.h,
+ * su2_observe.h) and the en_av partial layout and summation order are the
+ * single-lattice ones, so every lattice gives the same bits alone, in a
+ * batch, or in a served mix.
+ */
+#include "su2_batch.h"
+#include "su2_observe.h"
+
+namespace {
+
+/* index of the descriptor whose range [begin, next begin) holds vb */
+template <int FIELD>
+__device__ static inline uint32_t find_desc(const su2_bdesc *__restrict__ D, uint32_t n, uint32_t vb)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t b = FIELD == 0 ? D[mid].sw_begin : (FIELD == 1 ? D[mid].en_begin : D[mid].nm_begin);
+        if (b <= vb)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ static inline bool measure_now(const su2_bdesc &d, uint32_t s)
+{
+    if (s < (uint32_t)d.n_therm) return false;
+    const uint32_t k = s - (uint32_t)d.n_therm;
+    return (k % (uint32_t)d.n_cor) == (uint32_t)d.n_cor - 1u;
+}
+
+__device__ static inline bool normalize_pre(const su2_bdesc &d, uint32_t s)
+{
+    return s < (uint32_t)d.n_therm && ((s + 1u) % (uint32_t)d.nevery) == 0u;
+}
+
+__device__ static inline bool normalize_post(const su2_bdesc &d, uint32_t s)
+{
+    if (s + 1u == (uint32_t)d.n_therm) return true;
+    if (!measure_now(d, s)) return false;
+    const uint32_t i = (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor + 1u;
+    return (i % (uint32_t)d.nevery) == 0u;
+}
+
+template <int MU, int BC, typename LinkT>
+__device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t h)
+{
+    if (d.algo == 1)
+        su2_sweep_site<MU, BC, SU2_K_HB, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
+    else
+        su2_sweep_site<MU, BC, SU2_K_METRO, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
+}
+
+/* OR = 0: link update (heat-bath or Metropolis per lattice); OR = 1:
+ * over-relaxation pass number or_pass (lattices with fewer over-relaxation
+ * sweeps per tick skip it). */
+template <int MU, int OR, typename LinkT>
+__global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
+                                                int parity, int or_pass)
+{
+    const uint32_t n = H->n_active, nblk = H->sw_blocks;
+    for (uint32_t vb = blockIdx.x; vb < nblk; vb += gridDim.x) {
+        const su2_bdesc &d = D[find_desc<0>(D, n, vb)];
+        if (OR && or_pass >= d.or_sweeps) continue;
+        const uint32_t h = (vb - d.sw_begin) * blockDim.x + threadIdx.x;
+        if (h >= d.g.Vh) continue;
+        if (OR) {
+            switch (d.bc) {
+            case SU2_K_BC_PER:
+                su2_sweep_site<MU, SU2_K_BC_PER, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
+                break;
+            case SU2_K_BC_PCC:
+                su2_sweep_site<MU, SU2_K_BC_PCC, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
+                break;
+            default:
+                su2_sweep_site<MU, SU2_K_BC_PMC, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
+                break;
+            }
+        } else {
+            switch (d.bc) {
+            case SU2_K_BC_PER: site_link<MU, SU2_K_BC_PER, LinkT>(d, parity, h); break;
+            case SU2_K_BC_PCC: site_link<MU, SU2_K_BC_PCC, LinkT>(d, parity, h); break;
+            default: site_link<MU, SU2_K_BC_PMC, LinkT>(d, parity, h); break;
+            }
+        }
+    }
+}
+
+/* Reunitarisation of the lattices whose schedule asks for it this tick
+ * (post = 0: before the sweep, post = 1: after it); the post pass at the end
+ * of thermalisation also clears the lattice's acceptance counters. */
+template <typename LinkT>
+__global__ __launch_bounds__(256) void k_bnormalize(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
+                                                    const uint32_t *__restrict__ ctr, int post)
+{
+    const uint32_t n = H->n_active, nblk = H->nm_blocks;
+    for (uint32_t vb = blockIdx.x; vb < nblk; vb += gridDim.x) {
+        const su2_bdesc &d = D[find_desc<2>(D, n, vb)];
+        const uint32_t s = ctr[d.slot];
+        if (!(post ? normalize_post(d, s) : normalize_pre(d, s))) continue;
+        const uint32_t i = (vb - d.nm_begin) * blockDim.x + threadIdx.x;
+        /* acceptance restarts with the measurement (kernel.cu:190) */
+        if (post && s + 1u == (uint32_t)d.n_therm && i < 2u * d.g.Vh) d.accept[i] = 0;
+        if (i >= 6u * d.g.Vh) continue;
+        LinkT *U = (LinkT *)d.U + i;
+        su2_link_io<LinkT>::store(U, su2_q_normalize(su2_link_io<LinkT>::load(U)));
+    }
+}
+
+/* en_av pass 1 for the lattices that measure this tick; logical block b of
+ * a lattice has the single-lattice layout (p = b & 1, chunk b >> 1). */
+template <int PMC, typename LinkT>
+__device__ static inline void en_av_block(const su2_bdesc &d, uint32_t b)
+{
+    const uint32_t p = b & 1u;
+    const uint32_t h = (b >> 1) * blockDim.x + threadIdx.x;
+    double e = 0.0, pl = 0.0;
+    if (h < d.g.Vh) {
+        const uint32_t q = su2_div(h, d.g.div_lyh);
+        const uint32_t yh = h - q * d.g.Lyh;
+        const uint32_t t = su2_div(q, d.g.div_lx);
+        const uint32_t x = q - t * d.g.Lx;
+        su2_site_plaquettes<PMC, LinkT>((const LinkT *)d.U, d.g, p, h, t, x, yh, &e, &pl);
+    }
+    su2_block_sum2(e, pl);
+    if (threadIdx.x == 0) {
+        d.part[2 * b] = e;
+        d.part[2 * b + 1] = pl;
+    }
+}
+
+template <typename LinkT>
+__global__ __launch_bounds__(256) void k_ben_av_partial(const su2_bhead *__restrict__ H,
+                                                        const su2_bdesc *__restrict__ D,
+                                                        const uint32_t *__restrict__ ctr)
+{
+    const uint32_t n = H->n_active, nblk = H->en_blocks;
+    for (uint32_t vb = blockIdx.x; vb < nblk; vb += gridDim.x) {
+        const su2_bdesc &d = D[find_desc<1>(D, n, vb)];
+        if (!measure_now(d, ctr[d.slot])) continue;
+        const uint32_t b = vb - d.en_begin;
+        if (d.bc == SU2_K_BC_PMC)
+            en_av_block<1, LinkT>(d, b);
+        else
+            en_av_block<0, LinkT>(d, b);
+        __syncthreads(); /* su2_block_sum2's shared array is reused by the next block */
+    }
+}
+
+/* en_av pass 2: workgroup j (1024 threads) adds lattice j's partials in the
+ * single-lattice order and accumulates sum / V into its measurement slot. */
+__global__ __launch_bounds__(1024) void k_ben_av_final(const su2_bhead *__restrict__ H,
+                                                       const su2_bdesc *__restrict__ D,
+                                                       const uint32_t *__restrict__ ctr)
+{
+    const uint32_t n = H->n_active;
+    for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
+        const su2_bdesc &d = D[j];
+        const uint32_t s = ctr[d.slot];
+        if (!measure_now(d, s)) continue;
+        const uint32_t m = (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor;
+        double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t i = threadIdx.x; i < d.en_nb; i += 4 * blockDim.x) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t q = i + k * blockDim.x;
+                if (q < d.en_nb) {
+                    se[k] += d.part[2 * q];
+                    sp[k] += d.part[2 * q + 1];
+                }
+            }
+        }
+        double te = (se[0] + se[1]) + (se[2] + se[3]);
+        double tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+        su2_block_sum2(te, tp);
+        if (threadIdx.x == 0) {
+            const double invV = 1.0 / (2.0 * (double)d.g.Vh);
+            d.resE[m] += te * invV;
+            d.resP[m] += tp * invV;
+        }
+        __syncthreads();
+    }
+}
+
+/* End of a tick: every resident lattice has done one more sweep. */
+__global__ __launch_bounds__(256) void k_btick_end(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
+                                                   uint32_t *__restrict__ ctr)
+{
+    const uint32_t n = H->n_active;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) ctr[D[j].slot] += 1u;
+}
+
+template <int MU, int OR, typename LinkT>
+hipError_t launch_bsweep(const su2_bhead *H, const su2_bdesc *D, int parity, int or_pass, uint32_t grid,
+                         uint32_t block, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_bsweep<MU, OR, LinkT>), dim3(grid), dim3(block), 0, st, H, D, parity, or_pass);
+    return hipGetLastError();
+}
+
+template <int OR, typename LinkT>
+hipError_t launch_bsweep_mu(int mu, const su2_bhead *H, const su2_bdesc *D, int parity, int or_pass, uint32_t grid,
+                            uint32_t block, hipStream_t st)
+{
+    switch (mu) {
+    case 0: return launch_bsweep<0, OR, LinkT>(H, D, parity, or_pass, grid, block, st);
+    case 1: return launch_bsweep<1, OR, LinkT>(H, D, parity, or_pass, grid, block, st);
+    default: return launch_bsweep<2, OR, LinkT>(H, D, parity, or_pass, grid, block, st);
+    }
+}
+
+} // namespace
+
+hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, const su2_bhead *H, const su2_bdesc *D, int parity,
+                             int or_pass, uint32_t grid, uint32_t block, hipStream_t st)
+{
+    if (f32)
+        return over_relax ? launch_bsweep_mu<1, su2_l32>(mu, H, D, parity, or_pass, grid, block, st)
+                          : launch_bsweep_mu<0, su2_l32>(mu, H, D, parity, or_pass, grid, block, st);
+    return over_relax ? launch_bsweep_mu<1, su2_l64>(mu, H, D, parity, or_pass, grid, block, st)
+                      : launch_bsweep_mu<0, su2_l64>(mu, H, D, parity, or_pass, grid, block, st);
+}
+
+hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *ctr, int post,
+                                 uint32_t grid, hipStream_t st)
+{
+    if (f32)
+        hipLaunchKernelGGL(k_bnormalize<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, ctr, post);
+    else
+        hipLaunchKernelGGL(k_bnormalize<su2_l64>, dim3(grid), dim3(256), 0, st, H, D, ctr, post);
+    return hipGetLastError();
+}
+
+hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *ctr, uint32_t grid,
+                             uint32_t max_desc, hipStream_t st)
+{
+    if (f32)
+        hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, ctr);
+    else
+        hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(256), 0, st, H, D, ctr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ben_av_final, dim3(max_desc), dim3(1024), 0, st, H, D, ctr);
+    return hipGetLastError();
+}
+
+hipError_t su2_launch_btick_end(const su2_bhead *H, const su2_bdesc *D, uint32_t *ctr, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_btick_end, dim3(1), dim3(256), 0, st, H, D, ctr);
+    return hipGetLastError();
+}
+
+uint32_t su2_en_av_logical_blocks(const su2_kgeom &g) { return su2_grid_blocks(2ull * ((g.Vh + 255) / 256) * 256, 256); }

@@ -95,6 +95,15 @@ int sweep(const su2_lattice *l, void *phi, int prec, int *accept, uint32_t *stat
 
 } // namespace
 
+int su2_set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
 extern "C" {
 
 int su2_abi_version(void) { return SU2MC_ABI_VERSION; }

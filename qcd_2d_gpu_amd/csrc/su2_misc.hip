@@ -3,47 +3,12 @@
  * reunitarisation, the fused energy/plaquette reduction and the conversion
  * to and from the reference's on-disk field layout.
  */
-#include "su2_internal.h"
-#include "su2_update.h"
+#include "su2_observe.h"
 
 namespace {
 
 __constant__ uint32_t c_jump1[SU2_MRG_NJUMP * 9];
 __constant__ uint32_t c_jump2[SU2_MRG_NJUMP * 9];
-
-template <typename LinkT> struct io;
-template <> struct io<su2_l64> {
-    __device__ static inline su2_q load(const su2_l64 *p)
-    {
-        su2_l64 v = *p;
-        return su2_q_make(v.a0, v.a1, v.a2, v.a3);
-    }
-    __device__ static inline void store(su2_l64 *p, su2_q q)
-    {
-        su2_l64 v;
-        v.a0 = q.a0;
-        v.a1 = q.a1;
-        v.a2 = q.a2;
-        v.a3 = q.a3;
-        *p = v;
-    }
-};
-template <> struct io<su2_l32> {
-    __device__ static inline su2_q load(const su2_l32 *p)
-    {
-        su2_l32 v = *p;
-        return su2_q_make((double)v.a0, (double)v.a1, (double)v.a2, (double)v.a3);
-    }
-    __device__ static inline void store(su2_l32 *p, su2_q q)
-    {
-        su2_l32 v;
-        v.a0 = (float)q.a0;
-        v.a1 = (float)q.a1;
-        v.a2 = (float)q.a2;
-        v.a3 = (float)q.a3;
-        *p = v;
-    }
-};
 
 __device__ static inline uint32_t gtid(void) { return blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -72,7 +37,7 @@ __global__ __launch_bounds__(256) void k_init(LinkT *__restrict__ U, uint32_t *_
     const size_t Vh = g.Vh;
     if (START == 1) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) io<LinkT>::store(U + k * Vh + h, su2_q_id());
+        for (int k = 0; k < 6; ++k) su2_link_io<LinkT>::store(U + k * Vh + h, su2_q_id());
         return;
     }
     const uint32_t q = su2_div(h, g.div_lyh);
@@ -94,7 +59,7 @@ __global__ __launch_bounds__(256) void k_init(LinkT *__restrict__ U, uint32_t *_
                 l = su2_haar_link(s);
             else
                 l = su2_rand_link(s);
-            io<LinkT>::store(U + (size_t)(p * 3 + mu) * Vh + h, l);
+            su2_link_io<LinkT>::store(U + (size_t)(p * 3 + mu) * Vh + h, l);
         }
     }
 #pragma unroll
@@ -106,80 +71,7 @@ template <typename LinkT> __global__ __launch_bounds__(256) void k_normalize(Lin
 {
     const uint32_t i = gtid();
     if (i >= nlinks) return;
-    io<LinkT>::store(U + i, su2_q_normalize(io<LinkT>::load(U + i)));
-}
-
-/* The three plaquettes at site n of parity p (functions/observables.h:1-21),
- * each returned as 1 - 1/2 tr P:
- *   P_yx = U_y(n) U_x(n+y) U_y(n+x)^dag U_x(n)^dag
- *   P_yt = U_y(n) U_t(n+y) U_y(n+t)^dag U_t(n)^dag
- *   P_xt = U_x(n) U_t(n+x) U_x(n+t)^dag U_t(n)^dag
- * The reduction of each site (functions/observables.h:99-141) gives
- * E_site = -(1 - P_xt) and P_site = (3 - a - b - c) / 3; with PMC the y-x and
- * y-t plaquettes of y = Ly-1 count as 1 (a = b = 0).  (t, x, y') is the
- * decoded half-site shared by both parities. */
-template <int PMC, typename LinkT>
-__device__ static inline void site_plaquettes(const LinkT *__restrict__ U, const su2_kgeom &g, uint32_t p, uint32_t h,
-                                              uint32_t t, uint32_t x, uint32_t yh, double *e, double *pl)
-{
-    const size_t Vh = g.Vh;
-    const uint32_t y = 2u * yh + (((t + x) & 1u) ^ p);
-    const uint32_t yp = (y + 1u == g.Ly) ? 0u : y + 1u;
-    const uint32_t xp = (x + 1u == g.Lx) ? 0u : x + 1u;
-    const uint32_t tp = (t + 1u == g.Lt) ? 0u : t + 1u;
-    const uint32_t h_py = (yp >> 1) + x * g.Lyh + t * g.plane;
-    const uint32_t h_px = yh + xp * g.Lyh + t * g.plane;
-    const uint32_t h_pt = yh + x * g.Lyh + tp * g.plane;
-    const LinkT *Us = U + (size_t)(p * 3) * Vh;
-    const LinkT *Uo = U + (size_t)((1u - p) * 3) * Vh;
-
-    const su2_q uy = io<LinkT>::load(Us + 0 * Vh + h);
-    const su2_q ux = io<LinkT>::load(Us + 1 * Vh + h);
-    const su2_q ut = io<LinkT>::load(Us + 2 * Vh + h);
-    const su2_q ux_py = io<LinkT>::load(Uo + 1 * Vh + h_py);
-    const su2_q ut_py = io<LinkT>::load(Uo + 2 * Vh + h_py);
-    const su2_q uy_px = io<LinkT>::load(Uo + 0 * Vh + h_px);
-    const su2_q ut_px = io<LinkT>::load(Uo + 2 * Vh + h_px);
-    const su2_q uy_pt = io<LinkT>::load(Uo + 0 * Vh + h_pt);
-    const su2_q ux_pt = io<LinkT>::load(Uo + 1 * Vh + h_pt);
-
-    double a = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(uy, ux_py), uy_px), ux);
-    double b = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(uy, ut_py), uy_pt), ut);
-    double c = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(ux, ut_px), ux_pt), ut);
-    if (PMC && y == g.Ly - 1u) {
-        a = 0.0;
-        b = 0.0;
-    }
-    *e = -c;
-    *pl = (3.0 - a - b - c) / 3.0;
-}
-
-/* Deterministic block sum of two doubles: wavefront shuffles in a fixed
- * butterfly, then the wavefront totals in order (result in thread 0). */
-__device__ static inline void block_sum2(double &u, double &v)
-{
-    __shared__ double sh[2][16];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        u += __shfl_xor(u, off, 64);
-        v += __shfl_xor(v, off, 64);
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nw = (blockDim.x + 63) >> 6;
-    if (lane == 0) {
-        sh[0][wave] = u;
-        sh[1][wave] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double su = 0.0, sv = 0.0;
-        for (int w = 0; w < nw; ++w) {
-            su += sh[0][w];
-            sv += sh[1][w];
-        }
-        u = su;
-        v = sv;
-    }
+    su2_link_io<LinkT>::store(U + i, su2_q_normalize(su2_link_io<LinkT>::load(U + i)));
 }
 
 /* Pass 1 of en_av, one site per thread.  Logical block b covers the sites
@@ -200,9 +92,9 @@ __global__ __launch_bounds__(256) void k_en_av_partial(const LinkT *__restrict__
         const uint32_t yh = h - q * g.Lyh;
         const uint32_t t = su2_div(q, g.div_lx);
         const uint32_t x = q - t * g.Lx;
-        site_plaquettes<PMC, LinkT>(U, g, p, h, t, x, yh, &e, &pl);
+        su2_site_plaquettes<PMC, LinkT>(U, g, p, h, t, x, yh, &e, &pl);
     }
-    block_sum2(e, pl);
+    su2_block_sum2(e, pl);
     if (threadIdx.x == 0) {
         work[2 * b] = e;
         work[2 * b + 1] = pl;
@@ -229,7 +121,7 @@ __global__ __launch_bounds__(1024) void k_en_av_final(const double *__restrict__
     }
     double te = (se[0] + se[1]) + (se[2] + se[3]);
     double tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
-    block_sum2(te, tp);
+    su2_block_sum2(te, tp);
     if (threadIdx.x == 0) {
         *res += te * invV;
         *res2 += tp * invV;
@@ -244,7 +136,7 @@ __global__ __launch_bounds__(256) void k_to_ref(const LinkT *__restrict__ U, dou
     const uint32_t i = gtid();
     if (i >= 6u * Vh) return;
     const uint32_t pm = i / Vh, h = i - pm * Vh, p = pm / 3u, mu = pm - 3u * p;
-    su2_q l = io<LinkT>::load(U + i);
+    su2_q l = su2_link_io<LinkT>::load(U + i);
     double *d = phi + ((size_t)(p * Vh + h) * 3 + mu) * 4;
     d[0] = l.a0;
     d[1] = l.a1;
@@ -259,7 +151,7 @@ __global__ __launch_bounds__(256) void k_from_ref(const double *__restrict__ phi
     if (i >= 6u * Vh) return;
     const uint32_t pm = i / Vh, h = i - pm * Vh, p = pm / 3u, mu = pm - 3u * p;
     const double *d = phi + ((size_t)(p * Vh + h) * 3 + mu) * 4;
-    io<LinkT>::store(U + i, su2_q_make(d[0], d[1], d[2], d[3]));
+    su2_link_io<LinkT>::store(U + i, su2_q_make(d[0], d[1], d[2], d[3]));
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int *__restrict__ p, int v, size_t n)

@@ -1,0 +1,157 @@
+/*
+ * su2_sweep_site.h - the update of one link U_mu(n), n = half-site h of
+ * parity p, shared by the single-lattice launch (su2_sweep.hip) and the
+ * batched launch over many lattices (su2_batch.hip), so a lattice's links
+ * and random streams evolve bit for bit the same in both.
+ *
+ * Staple of U_mu(n) (functions/mc_updates.h:155-176 and the other five
+ * kernels), summed over nu != mu in the reference's order (nu ascending in
+ * y, x, t; the "positive" path before the "negative" one):
+ *   A += [U_nu(n+mu) U_mu(n+nu)^dag] U_nu(n)^dag
+ *   A += [U_nu(n+mu-nu)^dag U_mu(n-nu)^dag] U_nu(n-nu)
+ * Parities: n+mu, n+nu, n-nu lie on 1-p; n and n+mu-nu on p.
+ *
+ * Boundary conditions in y:
+ *   PCC (mc_updates_pcc.h:29,83,134,189): x and t links at y = 0 are not
+ *       updated (no draws, no acceptance count).
+ *   PMC (mc_updates_pmc.h:33,40,96,103,156,163,219,225): for the x and t
+ *       links the nu = y positive path is dropped at y = Ly-1 and the
+ *       negative path at y = 0.  y links use the periodic staple in both.
+ */
+#ifndef SU2_SWEEP_SITE_H
+#define SU2_SWEEP_SITE_H
+
+#include "su2_internal.h"
+#include "su2_update.h"
+
+template <typename LinkT> struct su2_link_io;
+
+template <> struct su2_link_io<su2_l64> {
+    __device__ static inline __attribute__((always_inline)) su2_q load(const su2_l64 *p)
+    {
+        su2_l64 v = *p;
+        return su2_q_make(v.a0, v.a1, v.a2, v.a3);
+    }
+    __device__ static inline __attribute__((always_inline)) void store(su2_l64 *p, su2_q q)
+    {
+        su2_l64 v;
+        v.a0 = q.a0;
+        v.a1 = q.a1;
+        v.a2 = q.a2;
+        v.a3 = q.a3;
+        *p = v;
+    }
+};
+
+template <> struct su2_link_io<su2_l32> {
+    __device__ static inline __attribute__((always_inline)) su2_q load(const su2_l32 *p)
+    {
+        su2_l32 v = *p;
+        return su2_q_make((double)v.a0, (double)v.a1, (double)v.a2, (double)v.a3);
+    }
+    __device__ static inline __attribute__((always_inline)) void store(su2_l32 *p, su2_q q)
+    {
+        su2_l32 v;
+        v.a0 = (float)q.a0;
+        v.a1 = (float)q.a1;
+        v.a2 = (float)q.a2;
+        v.a3 = (float)q.a3;
+        *p = v;
+    }
+};
+
+/* Half-site index of (t, x, y). */
+__device__ static inline __attribute__((always_inline)) uint32_t su2_hidx(const su2_kgeom &g, uint32_t t, uint32_t x,
+                                                                      uint32_t y)
+{
+    return (y >> 1) + x * g.Lyh + t * g.plane;
+}
+
+template <int MU, int BC, int UPD, typename LinkT>
+__device__ static inline __attribute__((always_inline)) void su2_sweep_site(LinkT *__restrict__ U,
+                                                                            uint32_t *__restrict__ rng,
+                                                                            int *__restrict__ accept,
+                                                                            const su2_kgeom &g, const su2_kphys &ph,
+                                                                            int parity, uint32_t h)
+{
+    typedef su2_link_io<LinkT> io;
+
+    const uint32_t q = su2_div(h, g.div_lyh);
+    const uint32_t yh = h - q * g.Lyh;
+    const uint32_t t = su2_div(q, g.div_lx);
+    const uint32_t x = q - t * g.Lx;
+    const uint32_t y = 2u * yh + (((t + x) & 1u) ^ (uint32_t)parity);
+
+    if (BC == SU2_K_BC_PCC && MU != SU2_DIR_Y && y == 0u) return;
+
+    /* wrapped neighbour coordinates */
+    const uint32_t yp = (y + 1u == g.Ly) ? 0u : y + 1u;
+    const uint32_t ym = (y == 0u) ? g.Ly - 1u : y - 1u;
+    const uint32_t xp = (x + 1u == g.Lx) ? 0u : x + 1u;
+    const uint32_t xm = (x == 0u) ? g.Lx - 1u : x - 1u;
+    const uint32_t tp = (t + 1u == g.Lt) ? 0u : t + 1u;
+    const uint32_t tm = (t == 0u) ? g.Lt - 1u : t - 1u;
+
+    const LinkT *Us = U + (size_t)(parity * 3) * g.Vh;       /* parity p */
+    const LinkT *Uo = U + (size_t)((1 - parity) * 3) * g.Vh; /* parity 1-p */
+    const size_t Vh = g.Vh;
+
+    /* coordinates of n + mu */
+    const uint32_t tmu = (MU == SU2_DIR_T) ? tp : t;
+    const uint32_t xmu = (MU == SU2_DIR_X) ? xp : x;
+    const uint32_t ymu = (MU == SU2_DIR_Y) ? yp : y;
+    const uint32_t h_pmu = su2_hidx(g, tmu, xmu, ymu);
+
+    su2_q A = su2_q_zero();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int NU = (k == 0) ? (MU == SU2_DIR_Y ? SU2_DIR_X : SU2_DIR_Y) : (MU == SU2_DIR_T ? SU2_DIR_X : SU2_DIR_T);
+        /* n + nu, n - nu, n + mu - nu */
+        const uint32_t h_pnu = su2_hidx(g, NU == SU2_DIR_T ? tp : t, NU == SU2_DIR_X ? xp : x, NU == SU2_DIR_Y ? yp : y);
+        const uint32_t h_mnu = su2_hidx(g, NU == SU2_DIR_T ? tm : t, NU == SU2_DIR_X ? xm : x, NU == SU2_DIR_Y ? ym : y);
+        const uint32_t h_pmu_mnu =
+            su2_hidx(g, NU == SU2_DIR_T ? tm : tmu, NU == SU2_DIR_X ? xm : xmu, NU == SU2_DIR_Y ? ym : ymu);
+
+        const su2_q u_nu_pmu = io::load(Uo + NU * Vh + h_pmu);
+        const su2_q u_mu_pnu = io::load(Uo + MU * Vh + h_pnu);
+        const su2_q u_nu_n = io::load(Us + NU * Vh + h);
+        const su2_q u_nu_pmu_mnu = io::load(Us + NU * Vh + h_pmu_mnu);
+        const su2_q u_mu_mnu = io::load(Uo + MU * Vh + h_mnu);
+        const su2_q u_nu_mnu = io::load(Uo + NU * Vh + h_mnu);
+
+        su2_q up = su2_q_mul_nd(su2_q_mul_nd(u_nu_pmu, u_mu_pnu), u_nu_n);
+        su2_q dn = su2_q_mul(su2_q_mul_dd(u_nu_pmu_mnu, u_mu_mnu), u_nu_mnu);
+        if (BC == SU2_K_BC_PMC && MU != SU2_DIR_Y && NU == SU2_DIR_Y) {
+            /* A is never -0 here, so adding +0 leaves it bit-identical. */
+            if (y == g.Ly - 1u) up = su2_q_zero();
+            if (y == 0u) dn = su2_q_zero();
+        }
+        A = su2_q_add(A, up);
+        A = su2_q_add(A, dn);
+    }
+
+    LinkT *dst = U + (size_t)(parity * 3 + MU) * g.Vh + h;
+    int acc;
+    if (UPD == SU2_K_OR) {
+        su2_q u = io::load(dst);
+        io::store(dst, su2_over_relax(A, u));
+        acc = 1;
+    } else {
+        uint32_t s[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * Vh + h];
+        if (UPD == SU2_K_HB) {
+            io::store(dst, su2_heat_bath(A, ph.beta, s));
+            acc = 1;
+        } else {
+            su2_q u = io::load(dst);
+            acc = su2_metropolis(A, &u, ph.beta, ph.eps, ph.hits, s);
+            io::store(dst, u);
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) rng[(size_t)i * Vh + h] = s[i];
+    }
+    accept[(size_t)parity * Vh + h] += acc;
+}
+
+#endif /* SU2_SWEEP_SITE_H */

@@ -1,0 +1,208 @@
+"""GPU tests of the batched engine and the server: every job's results are
+bit-identical to running that job alone through the single-lattice C ABI
+(the reference's run loop, kernel.cu:161-208), whatever else is resident,
+and whether jobs join, leave or the shapes are autotuned in between."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def run_alone(q, job, prec):
+    """The reference schedule on one lattice with the single-lattice entry
+    points; returns E, P series, final field (reference layout), RNG planes
+    and mean acceptance."""
+    g = q.Lattice(job.Lt, job.Lx, job.Ly, job.beta, job.epsilon, job.hits, precision=prec)
+    try:
+        g.setup_kernel(job.seed)
+        g.start(job.start, job.bc)
+        nev = job.normalize_every or 100
+
+        def sweep():
+            g.mc_up(job.bc, job.algo, 0)
+            for _ in range(job.or_sweeps):
+                g.mc_up(job.bc, job.algo, 1)
+
+        for i in range(1, job.n_therm + 1):
+            if i % nev == 0:
+                g.normalize()
+            sweep()
+        g.normalize()
+        g.accept_clear()
+        E, P = [], []
+        for i in range(1, job.n_meas + 1):
+            for _ in range(job.n_cor):
+                sweep()
+            if i % nev == 0:
+                g.normalize()
+            e, p = g.en_av(job.bc == 2)
+            E.append(e)
+            P.append(p)
+        acc = g.accept().astype(np.float64).sum()
+        hits = job.hits if job.algo == 0 else 1
+        n = job.n_meas * job.n_cor * 3 * (hits + job.or_sweeps)
+        acc = acc / g.V / n
+        if job.bc == 1:
+            acc = acc / ((job.Ly - 1) / job.Ly + 1 / job.Ly / 3)
+        return np.array(E), np.array(P), g.field_ref(), g.rng_state(), acc
+    finally:
+        g.close()
+
+
+def mixed_jobs(q, n, seed0=1000, small=True):
+    rng = np.random.default_rng(seed0)
+    jobs = []
+    for i in range(n):
+        L = int(rng.choice([4, 6, 8, 10])) if small else int(rng.choice(range(8, 49, 2)))
+        Ly = int(rng.choice([2, 4, 6, 8, 12])) if small else int(rng.choice(range(8, 65, 2)))
+        jobs.append(q.Job(L, L, Ly, float(np.float32(rng.uniform(2, 8))), bc=int(rng.integers(0, 3)),
+                          algo=int(rng.integers(0, 2)), start=int(rng.integers(0, 3)), seed=seed0 + i,
+                          n_therm=int(rng.integers(1, 5)), n_meas=int(rng.integers(2, 6)),
+                          n_cor=int(rng.integers(1, 3)), or_sweeps=int(rng.integers(0, 2)),
+                          epsilon=float(np.float32(rng.uniform(0.1, 0.6))), hits=int(rng.integers(1, 4)),
+                          normalize_every=int(rng.integers(2, 4))))
+    return jobs
+
+
+def check_same(q, eng, slot, job, prec, tag):
+    got = eng.read(slot, field=True, rng=True)
+    E, P, phi, st, acc = run_alone(q, job, prec)
+    assert np.array_equal(got["E"].view(np.uint64), E.view(np.uint64)), f"{tag}: energies differ"
+    assert np.array_equal(got["P"].view(np.uint64), P.view(np.uint64)), f"{tag}: plaquettes differ"
+    assert np.array_equal(got["field"].view(np.uint64), phi.view(np.uint64)), f"{tag}: links differ"
+    assert np.array_equal(got["rng"], st), f"{tag}: RNG states differ"
+    assert abs(got["accept"] - acc) <= 1e-12 * max(1.0, abs(acc)), f"{tag}: acceptance {got['accept']} vs {acc}"
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_equals_alone(gpu, prec, use_graph):
+    q = gpu
+    _say(f"[engine] batch vs alone prec={prec} graph={use_graph}")
+    jobs = mixed_jobs(q, 10, seed0=2000 + 10 * prec)
+    eng = q.Engine(precision=prec, max_jobs=32, budget_bytes=256 << 20, use_graph=use_graph)
+    try:
+        slots = [eng.add(j) for j in jobs]
+        eng.run()
+        for s, j in zip(slots, jobs):
+            d, t = eng.status(s)
+            assert d == t == j.n_therm + j.n_meas * j.n_cor
+            check_same(q, eng, s, j, prec, f"slot {s} {j}")
+    finally:
+        eng.close()
+
+
+def test_jobs_join_and_leave(gpu):
+    """Continuous batching on the engine: jobs added while others run, one
+    removed unfinished, results of the rest unchanged."""
+    q = gpu
+    _say("[engine] dynamic membership")
+    jobs = mixed_jobs(q, 8, seed0=3000)
+    eng = q.Engine(precision=0, max_jobs=8, budget_bytes=64 << 20, use_graph=True)
+    try:
+        first = [eng.add(j) for j in jobs[:4]]
+        eng.tick(2)
+        victim = first[1]
+        eng.remove(victim)
+        later = [eng.add(j) for j in jobs[4:]]
+        eng.tick(3)
+        eng.run()
+        for s, j in zip([first[0], first[2], first[3]] + later, [jobs[0], jobs[2], jobs[3]] + jobs[4:]):
+            check_same(q, eng, s, j, 0, f"slot {s}")
+    finally:
+        eng.close()
+
+
+def test_budget_admission(gpu):
+    q = gpu
+    _say("[engine] budget admission")
+    job = q.Job(16, 16, 16, 3.0, n_therm=2, n_meas=2)
+    need = q.load_library().su2_job_bytes(__import__("ctypes").byref(job.c()), 0)
+    eng = q.Engine(precision=0, max_jobs=4, budget_bytes=int(need * 2.5), use_graph=False)
+    try:
+        a = eng.add(job)
+        b = eng.add(q.Job(16, 16, 16, 3.0, seed=5, n_therm=2, n_meas=2))
+        with pytest.raises(q.Su2Error) as e:
+            eng.add(q.Job(16, 16, 16, 3.0, seed=6, n_therm=2, n_meas=2))
+        assert e.value.code == -7
+        eng.run()
+        eng.remove(a)
+        c = eng.add(q.Job(16, 16, 16, 3.0, seed=6, n_therm=2, n_meas=2))
+        eng.run()
+        check_same(q, eng, c, q.Job(16, 16, 16, 3.0, seed=6, n_therm=2, n_meas=2), 0, "readmitted")
+        used, cap = eng.memory()
+        assert used <= cap
+    finally:
+        eng.close()
+
+
+def test_autotune_keeps_results(gpu):
+    q = gpu
+    _say("[engine] autotune")
+    jobs = mixed_jobs(q, 6, seed0=4000)
+    eng = q.Engine(precision=0, max_jobs=8, budget_bytes=64 << 20, use_graph=True)
+    try:
+        slots = [eng.add(j) for j in jobs]
+        eng.tick(1)
+        b, g = eng.autotune()
+        assert b in (64, 128, 256) and g > 0
+        eng.run()
+        for s, j in zip(slots, jobs):
+            check_same(q, eng, s, j, 0, f"slot {s} after autotune")
+    finally:
+        eng.close()
+
+
+def test_server_continuous_batching(gpu):
+    """Requests submitted while others run are admitted as memory frees up;
+    each result equals the job run alone."""
+    q = gpu
+    _say("[server] continuous batching")
+    jobs = mixed_jobs(q, 12, seed0=5000)
+    need = max(q.load_library().su2_job_bytes(__import__("ctypes").byref(j.c()), 0) for j in jobs)
+    srv = q.Server(precision=0, max_jobs=16, budget_bytes=need * 3, use_graph=True)
+    try:
+        ids = []
+        for i, j in enumerate(jobs):
+            ids.append(srv.submit(j))
+            if i % 3 == 2:
+                time.sleep(0.002)
+        srv.drain()
+        series = {i: srv.series(i) for i in ids}
+        res = {r.id: r for r in srv.poll()}
+        assert sorted(res) == sorted(ids)
+        for i, j in zip(ids, jobs):
+            r = res[i]
+            assert r.status == 0
+            assert r.t_submit <= r.t_admit <= r.t_done
+            E, P, _, _, acc = run_alone(q, j, 0)
+            assert np.array_equal(series[i][0].view(np.uint64), E.view(np.uint64)), f"job {i} energies"
+            assert np.array_equal(series[i][1].view(np.uint64), P.view(np.uint64)), f"job {i} plaquettes"
+            assert abs(r.E_mean - E.mean()) < 1e-12 and abs(r.P_mean - P.mean()) < 1e-12
+            assert abs(r.accept - acc) < 1e-12
+    finally:
+        srv.close()
+
+
+def test_server_rejects_invalid_and_oversized(gpu):
+    q = gpu
+    _say("[server] invalid requests")
+    srv = q.Server(precision=0, max_jobs=4, budget_bytes=8 << 20, use_graph=False)
+    try:
+        with pytest.raises(q.Su2Error):
+            srv.submit(q.Job(15, 16, 16, 3.0))
+        with pytest.raises(q.Su2Error):
+            srv.submit(q.Job(64, 64, 64, 3.0))  # larger than the whole budget
+        jid = srv.submit(q.Job(4, 4, 4, 3.0, n_therm=1, n_meas=2))
+        srv.drain()
+        (r,) = srv.poll()
+        assert r.id == jid and r.status == 0
+    finally:
+        srv.close()
