@@ -46,6 +46,12 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=-1, help="CPU baseline steps (-1: auto, 0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--roofline-sweeps", type=int, default=20)
+    ap.add_argument("--batch-jobs", type=int, default=64, help="lattices of the batched ensemble (0: skip)")
+    ap.add_argument("--batch-therm", type=int, default=20)
+    ap.add_argument("--batch-meas", type=int, default=80)
+    ap.add_argument("--serve-requests", type=int, default=100, help="served requests (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--fp32-meas", type=int, default=1000, help="measurements of the FP32-link check (0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the run, launch the plain copy kernel 5 times on buffers of this size "
                          "(reference dispatches for calibrating rocprofv3 byte counters)")
@@ -182,6 +188,133 @@ def cpu_baseline(args, V_gpu):
                       f"{sec:.1f} s, OpenMP C restatement of the reference"}
 
 
+def ensemble_jobs(q, n, therm, meas):
+    """The batched-ensemble workload: Lt = Lx uniform over the even values
+    8..48, Ly uniform over the even values 8..64, beta uniform in [2, 8]
+    (rounded to float, as the reference stores it), boundary uniform over
+    periodic / PCC / PMC, heat-bath, hot start, seeds 1000 + i."""
+    rng = np.random.default_rng(20260101)
+    jobs = []
+    for i in range(n):
+        L = int(rng.choice(np.arange(8, 49, 2)))
+        Ly = int(rng.choice(np.arange(8, 65, 2)))
+        beta = float(np.float32(rng.uniform(2.0, 8.0)))
+        jobs.append(q.Job(L, L, Ly, beta, bc=int(rng.integers(0, 3)), algo=q.ALGO_HEATBATH, start=q.START_HOT,
+                          seed=1000 + i, n_therm=therm, n_meas=meas, n_cor=1))
+    return jobs
+
+
+def batched_bench(args, precision):
+    """All jobs in one engine (one persistent launch per direction x parity,
+    hipGraph-captured ticks, autotuned shape) against the same jobs run one
+    by one with the plain single-lattice kernels through the native driver
+    su2_run.  Results must be identical."""
+    import qcd_2d_gpu_amd as q
+
+    lib = q.load_library()
+    jobs = ensemble_jobs(q, args.batch_jobs, args.batch_therm, args.batch_meas)
+    need = sum(lib.su2_job_bytes(C.byref(j.c()), precision) for j in jobs)
+    eng = q.Engine(precision=precision, max_jobs=len(jobs), budget_bytes=int(need * 1.05) + (16 << 20),
+                   use_graph=True)
+    slots = [eng.add(j) for j in jobs]
+    eng.sync()
+    block, grid = eng.autotune()
+    t0 = time.perf_counter()
+    eng.run()
+    t_batch = time.perf_counter() - t0
+    series = {s: eng.read(s) for s in slots}
+    eng.close()
+    t_single = 0.0
+    same = True
+    for s, j in zip(slots, jobs):
+        r = q.run(Lt=j.Lt, Lx=j.Lx, Ly=j.Ly, beta=j.beta, boundary=j.bc, algo=j.algo, init=q.START_HOT,
+                  N=j.n_meas, N_term=j.n_therm, seed=j.seed, precision=precision, save=1, sconf=1)
+        t_single += r.seconds_therm + r.seconds_meas
+        same &= bool(np.array_equal(r.energy.view(np.uint64), series[s]["E"].view(np.uint64)) and
+                     np.array_equal(r.plaquette.view(np.uint64), series[s]["P"].view(np.uint64)))
+    ups = sum(j.link_updates for j in jobs)
+    return {"jobs": len(jobs), "sites": int(sum(j.V for j in jobs)), "sweeps_per_job": args.batch_therm + args.batch_meas,
+            "link_updates": ups, "batched_s": t_batch, "one_by_one_s": t_single,
+            "batched_link_updates_per_s": ups / t_batch, "one_by_one_link_updates_per_s": ups / t_single,
+            "speedup": t_single / t_batch, "autotuned_block": block, "autotuned_grid": grid,
+            "results_identical": same, "precision": "FP64" if precision == 0 else "FP32 links"}
+
+
+def serving_bench(args):
+    """Poisson stream of run requests (Lt = Lx in {16,24,32,48,64},
+    Ly in {16,32,64}, beta uniform in [2, 8], random boundary, heat-bath,
+    50 thermalisation + 200 measured sweeps) into the continuous-batching
+    server with a 200 GB budget."""
+    import threading
+
+    import qcd_2d_gpu_amd as q
+
+    rng = np.random.default_rng(555)
+    n = args.serve_requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=n))
+    jobs = [q.Job(int(L), int(L), int(Ly), float(np.float32(b)), bc=int(bc), algo=q.ALGO_HEATBATH,
+                  start=q.START_HOT, seed=7000 + i, n_therm=50, n_meas=200)
+            for i, (L, Ly, b, bc) in enumerate(zip(rng.choice([16, 24, 32, 48, 64], n), rng.choice([16, 32, 64], n),
+                                                   rng.uniform(2.0, 8.0, n), rng.integers(0, 3, n)))]
+    srv = q.Server(precision=q.PREC_F64, max_jobs=256, budget_bytes=200 * 10 ** 9, use_graph=True)
+    # warm the server's kernels and graph with one small request
+    srv.submit(q.Job(8, 8, 8, 3.0, n_therm=2, n_meas=2))
+    srv.drain()
+    srv.poll()
+    base = srv.now()
+
+    def feed():
+        for t, j in zip(arrivals, jobs):
+            d = base + t - srv.now()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(j)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    srv.drain()
+    res = [r for r in srv.poll() if r.status == 0]
+    srv.close()
+    lat = np.array([r.t_done - r.t_submit for r in res])
+    t_first = min(r.t_submit for r in res)
+    t_last = max(r.t_done for r in res)
+    ups = sum(r.link_updates for r in res)
+    return {"requests": n, "completed": len(res), "offered_rate_per_s": args.serve_rate,
+            "jobs_per_s": len(res) / (t_last - t_first), "link_updates_per_s": ups / (t_last - t_first),
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "window_s": t_last - t_first}
+
+
+def fp32_check(args):
+    """The boundary + Metropolis + precision configuration: Lt = Lx = 32,
+    Ly = 48, PCC and PMC, Metropolis eps 0.5, 4 hits, beta 4.0, hot start, in
+    FP64 and with FP32 link storage; plaquette difference in units of the
+    combined binned error."""
+    import qcd_2d_gpu_amd as q
+
+    out = {}
+    runs = {}
+    for prec in (q.PREC_F64, q.PREC_F32):
+        eng = q.Engine(precision=prec, max_jobs=2, budget_bytes=256 << 20, use_graph=True)
+        slots = {bc: eng.add(q.Job(32, 32, 48, 4.0, bc=bc, algo=q.ALGO_METROPOLIS, start=q.START_HOT, seed=4321,
+                                   n_therm=200, n_meas=args.fp32_meas, epsilon=0.5, hits=4))
+                 for bc in (q.BC_PCC, q.BC_PMC)}
+        eng.run()
+        for bc, s in slots.items():
+            runs[(prec, bc)] = eng.read(s)
+        eng.close()
+    for bc, name in ((q.BC_PCC, "pcc"), (q.BC_PMC, "pmc")):
+        P64, P32 = runs[(0, bc)]["P"], runs[(1, bc)]["P"]
+        m64, e64 = q.stats.binned_error(P64, 20)
+        m32, e32 = q.stats.binned_error(P32, 20)
+        sig = float(np.hypot(e64, e32))
+        out[name] = {"P_fp64": m64, "err_fp64": e64, "P_fp32": m32, "err_fp32": e32,
+                     "diff": m32 - m64, "diff_over_sigma": (m32 - m64) / sig if sig else 0.0,
+                     "accept_fp64": runs[(0, bc)]["accept"], "accept_fp32": runs[(1, bc)]["accept"]}
+    return out
+
+
 def load_traffic():
     """HBM bytes per heat-bath launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present:
@@ -209,6 +342,14 @@ def main():
     hb_gbs = BYTES_HB * per_launch_updates / (r["hb_ms"] * 1e-3) / 1e9
     or_gbs = BYTES_OR * per_launch_updates / (r["or_ms"] * 1e-3) / 1e9
     step_bytes = 3 * V * (BYTES_HB + BYTES_OR) + 96.0 * V
+    extra = {}
+    if args.batch_jobs:
+        extra["batched_ensemble_fp64"] = batched_bench(args, 0)
+        extra["batched_ensemble_fp32"] = batched_bench(args, 1)
+    if args.serve_requests:
+        extra["serving"] = serving_bench(args)
+    if args.fp32_meas:
+        extra["fp32_links_vs_fp64"] = fp32_check(args)
     cpu = None
     if args.cpu_steps != 0:
         cpu = cpu_baseline(args, V)
@@ -241,6 +382,7 @@ def main():
         "observables": {"energy": r["E"], "plaquette": r["P"]},
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

@@ -230,7 +230,7 @@ typedef struct su2_engine_config {
     int device;
     int max_jobs;        /* resident jobs (0: 256) */
     int max_or;          /* over-relaxation passes per tick (0: 1) */
-    size_t budget_bytes; /* device memory for job data (0: 1 GiB) */
+    size_t budget_bytes; /* device memory for job data (0: 1 GiB; at most 90 % of the free memory) */
     int use_graph;       /* replay ticks from a captured hipGraph */
     int block;           /* sweep workgroup size: 64, 128 or 256 (0: autotuned) */
     int grid;            /* persistent workgroups per launch (0: autotuned) */

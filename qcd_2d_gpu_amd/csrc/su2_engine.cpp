@@ -324,9 +324,14 @@ int su2_engine_create(const su2_engine_config *cfg, su2_engine **out)
         return code;
     };
     if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) return bail(SU2_EHIP);
-    if (hipMalloc((void **)&e->arena, e->cfg.budget_bytes) != hipSuccess)
-        return bail(su2_set_error(SU2_ENOMEM, "arena of %zu bytes", e->cfg.budget_bytes));
-    e->cap = e->cfg.budget_bytes;
+    /* the arena is the budget, or 90 % of the free device memory if that is
+     * smaller */
+    size_t freeb = 0, totalb = 0, cap = e->cfg.budget_bytes;
+    if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && freeb > 0) cap = std::min(cap, freeb / 10 * 9);
+    cap &= ~(size_t)255;
+    if (hipMalloc((void **)&e->arena, cap) != hipSuccess)
+        return bail(su2_set_error(SU2_ENOMEM, "arena of %zu bytes", cap));
+    e->cap = cap;
     e->free_[0] = e->cap;
     e->slots.resize((size_t)e->cfg.max_jobs);
     if (hipMalloc((void **)&e->dH, sizeof(su2_bhead)) != hipSuccess ||
