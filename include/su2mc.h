@@ -310,9 +310,11 @@ typedef struct su2_run_config {
     float beta;         /* betal */
     int boundary;       /* Boundary: SU2_BC_* */
     int algo;           /* ALGO: SU2_ALGO_* */
-    int readconf;       /* 0 new start, 1 read Config_ then thermalise, 2 read without thermalisation */
+    int readconf;       /* 0 new start, 1 read Config_ then thermalise, 2 read without thermalisation;
+                           the RNG state is restored from Config_*_rng.bin when present (extension),
+                           else reseeded from seed as the reference does */
     int init;           /* SU2_START_* (init: 0 hot, 1 cold; 2 Haar extension) */
-    int sconf;          /* 0: save the configuration after thermalisation */
+    int sconf;          /* 0: save the configuration (and, as an extension, the RNG state) after thermalisation */
     int save;           /* 0: write the result files */
     int N;              /* measurements */
     int N_term;         /* thermalisation sweeps */

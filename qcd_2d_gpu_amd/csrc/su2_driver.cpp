@@ -176,6 +176,32 @@ int read_conf(const su2_run_config *c, std::vector<double> &phi, const su2_latti
     return got == 2 ? SU2_OK : SU2_EIO;
 }
 
+/* RNG checkpoint (extension: the reference reseeds on resume, so its
+ * resumed runs are not reproducible): Config_<tail>_rng.bin holds the raw
+ * uint32 state planes g1[0..2], g2[0..2] of the V/2 streams. */
+int write_rng(const su2_run_config *c, const std::vector<uint32_t> &st, const su2_lattice &l, const char *bound)
+{
+    names n = make_tail(l, bound, c->nconfs);
+    std::string f = path_of(c, (std::string("Config_") + n.tail + "_rng.bin").c_str());
+    FILE *fp = fopen(f.c_str(), "wb");
+    if (!fp) return SU2_EIO;
+    size_t w = fwrite(st.data(), sizeof(uint32_t), st.size(), fp);
+    fclose(fp);
+    return w == st.size() ? SU2_OK : SU2_EIO;
+}
+
+/* 1: restored, 0: no checkpoint file, < 0: error */
+int read_rng(const su2_run_config *c, std::vector<uint32_t> &st, const su2_lattice &l, const char *bound)
+{
+    names n = make_tail(l, bound, c->nconfr);
+    std::string f = path_of(c, (std::string("Config_") + n.tail + "_rng.bin").c_str());
+    FILE *fp = fopen(f.c_str(), "rb");
+    if (!fp) return 0;
+    size_t r = fread(st.data(), sizeof(uint32_t), st.size(), fp);
+    fclose(fp);
+    return r == st.size() ? 1 : SU2_EIO;
+}
+
 double mean_det(const std::vector<double> &phi)
 {
     /* print_det (lattice_setup.h:176-183) */
@@ -283,6 +309,10 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
         RUN_CHECK(read_conf(c, h_phi, l, &Nterms, &inits, bound));
         RUN_CHECK(su2_memcpy_h2d(ref.p, h_phi.data(), h_phi.size() * sizeof(double), stream));
         RUN_CHECK(su2_field_from_ref(&l, (const double *)ref.p, phi.p, prec, stream));
+        std::vector<uint32_t> hst((size_t)l.V / 2 * 6);
+        int got = read_rng(c, hst, l, bound);
+        if (got < 0) return got;
+        if (got == 1) RUN_CHECK(su2_memcpy_h2d(st.p, hst.data(), hst.size() * sizeof(uint32_t), stream));
     }
     RUN_CHECK(su2_field_to_ref(&l, phi.p, prec, (double *)ref.p, stream));
     RUN_CHECK(su2_memcpy_d2h(h_phi.data(), ref.p, h_phi.size() * sizeof(double), stream));
@@ -333,6 +363,9 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
             RUN_CHECK(su2_field_to_ref(&l, phi.p, prec, (double *)ref.p, stream));
             RUN_CHECK(su2_memcpy_d2h(h_phi.data(), ref.p, h_phi.size() * sizeof(double), stream));
             RUN_CHECK(write_conf(c, h_phi, l, Nterms, inits, bound));
+            std::vector<uint32_t> hst((size_t)l.V / 2 * 6);
+            RUN_CHECK(su2_memcpy_d2h(hst.data(), st.p, hst.size() * sizeof(uint32_t), stream));
+            RUN_CHECK(write_rng(c, hst, l, bound));
         }
     }
     RUN_CHECK(su2_stream_synchronize(stream));

@@ -1,0 +1,121 @@
+"""GPU tests of the host driver su2_run (kernel.cu:49-268): the measurement
+series against the CPU checker run on the reference's loop, the result
+files in the reference's text formats (functions/output.h), checkpoint and
+bit-exact resume, acceptance normalisation, and the su2mc executable."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+def oracle_run(O, Lt, Lx, Ly, beta, bc, algo, init, N_term, N, seed, eps=0.1, upd=1):
+    o = O.Oracle(Lt, Lx, Ly, beta, eps, upd)
+    o.setup_kernel(seed)
+    o.start(init, bc)
+    for i in range(1, N_term + 1):
+        if i % 100 == 0:
+            o.normalize()
+        o.mc_up(bc, algo, 0)
+    o.normalize()
+    after_therm = o.phi.copy()
+    o.accept[:] = 0
+    E, P = [], []
+    for i in range(1, N + 1):
+        o.mc_up(bc, algo, 0)
+        if i % 100 == 0:
+            o.normalize()
+        e, p = o.en_av(bc == 2)
+        E.append(e)
+        P.append(p)
+    return np.array(E), np.array(P), after_therm, o
+
+
+def close(a, b):
+    return np.all(np.abs(a - b) <= REL * np.maximum(np.abs(b), 1e-300))
+
+
+def test_c1_run_matches_checker_and_writes_reference_files(gpu, oracle_mod, tmp_path):
+    q = gpu
+    print("[driver] C1 16^3 cold beta 3 seed 1234", flush=True)
+    r = q.run(Lt=16, Lx=16, Ly=16, beta=3.0, boundary=0, algo=1, init=q.START_COLD, N=100, N_term=100, seed=1234,
+              save=0, sconf=0, outdir=str(tmp_path))
+    E, P, after_therm, o = oracle_run(oracle_mod, 16, 16, 16, 3.0, 0, 1, 1, 100, 100, 1234)
+    assert close(r.energy, E) and close(r.plaquette, P)
+    assert r.accept == 1.0 and r.Nterms == 100
+    assert abs(r.det_mean - 1.0) < 1e-14
+    tail = "16_16_16_B_3.00_per_1"
+    conf = np.fromfile(tmp_path / f"Config_{tail}.txt", dtype=np.float64)
+    assert np.array_equal(conf.view(np.uint64), after_therm.view(np.uint64))
+    assert (tmp_path / f"Config_{tail}_data.txt").read_text() == "N_termalization 100 Type_termalization 1"
+    m, v = q.stats.mean_variance(r.energy)
+    lines = (tmp_path / f"Energy_{tail}.txt").read_text().splitlines()
+    assert lines[0] == ("N_termalizacion 100 N 100 N_correlation 1  D_t 16 D_x 16 D_ y 16 deltaphi 0.10 "
+                        "accept 1.00")
+    assert lines[1] == "M Energy Error"
+    assert lines[2] == f"3.000000 {m:.10f} {np.sqrt(v / 99):.10f}"
+    chk = (tmp_path / f"Check_{tail}.txt").read_text().splitlines()
+    assert chk[1] == "I Energy Avpl" and len(chk) == 102
+    for i in (0, 57, 99):
+        assert chk[i + 2] == f"{i} {r.energy[i]:.10f} {r.plaquette[i]:.10f}"
+    av = (tmp_path / f"Energy_av_{tail}.txt").read_text().splitlines()
+    assert len(av) == 4 and av[1] == "M Energy/Avplaquette Error"
+    ac = (tmp_path / f"Autocorrelation_{tail}.txt").read_text().splitlines()
+    assert len(ac) == 82 and ac[1] == "N Autocorrelation"
+    rho = q.stats.autocorrelation(r.energy, 80, m, v)
+    assert ac[2] == f"1 {rho[0]:.10f}"
+
+
+def test_resume_from_checkpoint_is_bit_exact(gpu, tmp_path):
+    q = gpu
+    print("[driver] checkpoint/resume", flush=True)
+    a = tmp_path / "a"
+    a.mkdir()
+    kw = dict(Lt=8, Lx=10, Ly=12, beta=2.5, boundary=2, algo=0, init=q.START_HOT, epsilon=0.4, upd=3, seed=77)
+    full = q.run(N_term=60, N=40, sconf=0, save=1, outdir=str(a), **kw)
+    resumed = q.run(N_term=0, N=40, readconf=2, save=1, outdir=str(a), **kw)
+    assert np.array_equal(full.energy.view(np.uint64), resumed.energy.view(np.uint64))
+    assert np.array_equal(full.plaquette.view(np.uint64), resumed.plaquette.view(np.uint64))
+    assert resumed.Nterms == 60
+    # without the RNG file the run reseeds (the reference's behaviour) and diverges
+    os.remove(a / "Config_8_10_12_B_2.50_pmc_1_rng.bin")
+    reseeded = q.run(N_term=0, N=40, readconf=2, save=1, outdir=str(a), **kw)
+    assert not np.array_equal(reseeded.energy, full.energy)
+
+
+@pytest.mark.parametrize("bc", [0, 1, 2])
+def test_acceptance_normalisation(gpu, bc):
+    q = gpu
+    print(f"[driver] acceptance bc={bc}", flush=True)
+    hb = q.run(Lt=8, Lx=8, Ly=10, beta=3.0, boundary=bc, algo=1, init=q.START_HOT, N=20, N_term=10, seed=3)
+    assert abs(hb.accept - 1.0) < 1e-12  # heat-bath always accepts; PCC rescaled (kernel.cu:226-228)
+    me = q.run(Lt=8, Lx=8, Ly=10, beta=3.0, boundary=bc, algo=0, init=q.START_HOT, N=20, N_term=10, seed=3,
+               epsilon=0.5, upd=4)
+    assert 0.05 < me.accept < 0.95
+
+
+def test_fp32_links_run(gpu):
+    q = gpu
+    print("[driver] FP32 links", flush=True)
+    r = q.run(Lt=16, Lx=16, Ly=16, beta=4.0, boundary=0, algo=1, init=q.START_HOT, N=50, N_term=50, seed=9,
+              precision=q.PREC_F32)
+    assert abs(r.det_mean - 1.0) < 1e-6
+    assert 0.5 < r.P_mean < 0.9
+
+
+def test_cli(gpu, tmp_path):
+    exe = os.path.join(os.path.dirname(gpu.__file__), "su2mc")
+    print("[driver] su2mc executable", flush=True)
+    out = subprocess.run([exe, "--Lt", "8", "--Lx", "8", "--Ly", "8", "--beta", "3", "--init", "1", "--N", "20",
+                          "--N_term", "10", "--seed", "5", "--save", "0", "--outdir", str(tmp_path)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "Plaquette" in out.stdout and "accept 1.000000" in out.stdout
+    assert (tmp_path / "Check_8_8_8_B_3.00_per_1.txt").exists()
+    bad = subprocess.run([exe, "--Lt", "7"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "even" in bad.stderr
