@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=-1, help="CPU baseline steps (-1: auto, 0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--roofline-sweeps", type=int, default=20)
+    ap.add_argument("--large-L", type=int, default=256, help="extent of the DRAM-bound timing (0: skip)")
     ap.add_argument("--batch-jobs", type=int, default=64, help="lattices of the batched ensemble (0: skip)")
     ap.add_argument("--batch-therm", type=int, default=20)
     ap.add_argument("--batch-meas", type=int, default=80)
@@ -186,6 +187,51 @@ def cpu_baseline(args, V_gpu):
     return {"value": ups, "unit": "link updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps (heat-bath + over-relaxation sweep) of the same {L}^3 lattice, "
                       f"{sec:.1f} s, OpenMP C restatement of the reference"}
+
+
+def large_lattice_bench(args):
+    """Heat-bath and over-relaxation launches at L^3 (256^3: 1.6 GB of FP64
+    links, far beyond the 256 MB Infinity Cache, so DRAM-bound), timed per
+    launch with HIP events on the launch stream."""
+    import qcd_2d_gpu_amd as q
+    from qcd_2d_gpu_amd import _lib as L
+
+    lib = q.load_library()
+    stream = C.c_void_p()
+    L.check(lib.su2_stream_create(C.byref(stream)), "stream_create")
+    n = args.large_L
+    g = q.Lattice(n, n, n, 5.0, precision=q.PREC_F64, stream=stream)
+    g.setup_kernel(7)
+    g.rand_init2()
+    for _ in range(2):
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
+    out = {"L": n, "sites": g.V}
+    for upd, name, nbytes in ((q.UPDATE_LINK, "heatbath", BYTES_HB), (q.UPDATE_OVERRELAX, "overrelax", BYTES_OR)):
+        evs = []
+        for _ in range(3):
+            for mu in range(3):
+                for par in range(2):
+                    a, b = C.c_void_p(), C.c_void_p()
+                    lib.su2_event_create(C.byref(a))
+                    lib.su2_event_create(C.byref(b))
+                    lib.su2_event_record(a, stream)
+                    g.mc_update(0, q.ALGO_HEATBATH, upd, mu, par)
+                    lib.su2_event_record(b, stream)
+                    evs.append((a, b))
+        g.synchronize()
+        ms, v = C.c_float(), []
+        for a, b in evs:
+            lib.su2_event_elapsed_ms(a, b, C.byref(ms))
+            v.append(ms.value * 1e-3)
+            lib.su2_event_destroy(a)
+            lib.su2_event_destroy(b)
+        t = float(np.mean(v))
+        out[name] = {"avg_launch_us": t * 1e6, "algorithmic_gbs": nbytes * g.Vh / t / 1e9,
+                     "frac_of_hbm_peak": nbytes * g.Vh / t / 1e9 / HBM_PEAK_GBS}
+    g.close()
+    lib.su2_stream_destroy(stream)
+    return out
 
 
 def ensemble_jobs(q, n, therm, meas):
@@ -343,6 +389,8 @@ def main():
     or_gbs = BYTES_OR * per_launch_updates / (r["or_ms"] * 1e-3) / 1e9
     step_bytes = 3 * V * (BYTES_HB + BYTES_OR) + 96.0 * V
     extra = {}
+    if args.large_L:
+        extra["large_lattice_fp64"] = large_lattice_bench(args)
     if args.batch_jobs:
         extra["batched_ensemble_fp64"] = batched_bench(args, 0)
         extra["batched_ensemble_fp32"] = batched_bench(args, 1)

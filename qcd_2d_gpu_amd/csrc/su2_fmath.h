@@ -99,16 +99,11 @@ SU2_FM_FN float su2_fmaxf(float a, float b)
     return (a > b) ? a : ((b > a) ? b : (a == 0.0f ? (a + b) : a));
 }
 
-/* Natural logarithm for finite x > 0. */
-SU2_FM_FN double su2_log_d(double x)
+/* Natural logarithm of a normal, finite double x > 0. */
+SU2_FM_FN double su2_log_normal_d(double x)
 {
     uint64_t u = su2_fm_bits(x);
-    int e = (int)((u >> 52) & 0x7ff);
-    if (e == 0) { /* subnormal: scale into the normal range */
-        u = su2_fm_bits(x * 0x1p54);
-        e = (int)((u >> 52) & 0x7ff) - 54;
-    }
-    e -= 1023;
+    int e = (int)((u >> 52) & 0x7ff) - 1023;
     double m = su2_fm_from_bits((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
     if (m > SU2_SQRT2) {
         m = m * 0.5;
@@ -128,12 +123,17 @@ SU2_FM_FN double su2_log_d(double x)
     return de * SU2_LN2_HI + (de * SU2_LN2_LO + logm);
 }
 
-/* logf of the reference (functions/mc_updates.h:49) for x > 0. */
-SU2_FM_FN float su2_logf(float x)
+/* Natural logarithm for finite x > 0 (subnormals scaled into range). */
+SU2_FM_FN double su2_log_d(double x)
 {
-    if (x == 0.0f) return -__builtin_inff();
-    return (float)su2_log_d((double)x);
+    if (su2_fm_bits(x) < 0x0010000000000000ULL) return su2_log_normal_d(x * 0x1p54) - 54.0 * SU2_LN2_HI - 54.0 * SU2_LN2_LO;
+    return su2_log_normal_d(x);
 }
+
+/* logf of the reference (functions/mc_updates.h:49) for finite x > 0.
+ * Every positive float, subnormals included, is a normal double, so no
+ * range scaling is needed; the heat-bath calls it on draws in (0, 1]. */
+SU2_FM_FN float su2_logf(float x) { return (float)su2_log_normal_d((double)x); }
 
 /* Exponential in double. */
 SU2_FM_FN double su2_exp_d(double x)
