@@ -33,12 +33,21 @@ struct su2_bhead {
     uint32_t nm_blocks;
 };
 
-hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, const su2_bhead *H, const su2_bdesc *D, int parity,
-                             int or_pass, uint32_t grid, uint32_t block, hipStream_t st);
-hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *ctr, int post,
-                                 uint32_t grid, hipStream_t st);
-hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *ctr, uint32_t grid,
-                             uint32_t max_desc, hipStream_t st);
+/* The first virtual block of every descriptor, per launch kind, as compact
+ * arrays (sw, en, nm; max_desc entries each) that workgroups stage in LDS
+ * for the block -> lattice search. */
+#define SU2_BATCH_LDS_DESC 1024
+
+/* algorithm classes of the resident jobs for the update launches */
+enum { SU2_BALGO_METRO = 0, SU2_BALGO_HB = 1, SU2_BALGO_MIXED = 2 };
+
+hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, int algo_class, const su2_bhead *H,
+                             const su2_bdesc *D, const uint32_t *B, uint32_t max_desc, int parity, int or_pass,
+                             uint32_t grid, uint32_t block, hipStream_t st);
+hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B,
+                                 uint32_t max_desc, const uint32_t *ctr, int post, uint32_t grid, hipStream_t st);
+hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, uint32_t max_desc,
+                             const uint32_t *ctr, uint32_t grid, hipStream_t st);
 hipError_t su2_launch_btick_end(const su2_bhead *H, const su2_bdesc *D, uint32_t *ctr, hipStream_t st);
 uint32_t su2_en_av_logical_blocks(const su2_kgeom &g);
 

@@ -8,11 +8,12 @@
  * su2_bdesc, one per resident lattice).  Every lattice is cut into virtual
  * blocks; block ranges are consecutive in the descriptor order, and a
  * launch's workgroups walk the virtual blocks with a grid stride, finding
- * their lattice by binary search on the range starts.  A virtual block
- * never spans two lattices, so the boundary condition and algorithm are
- * uniform in a workgroup.  Because the launch geometry does not depend on
- * the work list, one captured hipGraph stays valid when lattices join or
- * leave: only the descriptor table is rewritten between graph launches.
+ * their lattice by binary search on the range starts (compact arrays of
+ * them, staged in LDS).  A virtual block never spans two lattices, so the
+ * boundary condition and algorithm are uniform in a workgroup.  Because the
+ * launch geometry does not depend on the work list, a captured hipGraph
+ * stays valid when lattices join or leave: only the descriptor table is
+ * rewritten between graph launches.
  *
  * Each lattice advances by its own schedule, the reference's run loop
  * (kernel.cu:161-208): with s = sweeps done before this tick,
@@ -31,20 +32,53 @@
 
 namespace {
 
-/* index of the descriptor whose range [begin, next begin) holds vb */
-template <int FIELD>
-__device__ static inline uint32_t find_desc(const su2_bdesc *__restrict__ D, uint32_t n, uint32_t vb)
+/* The virtual blocks [*vb0, *vend) with stride *step that this workgroup
+ * walks.  Workgroups are dispatched to the 8 XCDs round-robin by blockIdx,
+ * so each XCD gets one contiguous eighth of the virtual blocks: tiles of a
+ * lattice that share neighbour links meet in the same L2. */
+__device__ static inline void xcd_walk(uint32_t nblk, uint32_t *vb0, uint32_t *vend, uint32_t *step)
+{
+    const uint32_t G = gridDim.x;
+    if (G & 7u) {
+        *vb0 = blockIdx.x;
+        *vend = nblk;
+        *step = G;
+        return;
+    }
+    const uint32_t per = (nblk + 7u) >> 3, x = blockIdx.x & 7u;
+    *vb0 = x * per + (blockIdx.x >> 3);
+    *vend = min(nblk, (x + 1u) * per);
+    *step = G >> 3;
+}
+
+/* Stages one launch kind's range starts in LDS when they fit (the search
+ * then costs no dependent global loads); returns whether they did. */
+__device__ static inline bool stage_begins(const uint32_t *__restrict__ Bk, uint32_t n, uint32_t *s_begin)
+{
+    const bool lds = n <= SU2_BATCH_LDS_DESC;
+    if (lds)
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_begin[i] = Bk[i];
+    __syncthreads();
+    return lds;
+}
+
+/* Index of the descriptor whose range [begin, next begin) holds vb.  vb is
+ * uniform in the workgroup, so the result is too; readfirstlane says so to
+ * the compiler, which then reads the descriptor with scalar loads into
+ * SGPRs instead of one copy per lane in VGPRs. */
+__device__ static inline uint32_t find_desc(const uint32_t *__restrict__ Bk, const uint32_t *s_begin, bool lds,
+                                            uint32_t n, uint32_t vb)
 {
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t b = FIELD == 0 ? D[mid].sw_begin : (FIELD == 1 ? D[mid].en_begin : D[mid].nm_begin);
+        const uint32_t b = lds ? s_begin[mid] : Bk[mid];
         if (b <= vb)
             lo = mid;
         else
             hi = mid;
     }
-    return lo;
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
 }
 
 __device__ static inline bool measure_now(const su2_bdesc &d, uint32_t s)
@@ -67,25 +101,31 @@ __device__ static inline bool normalize_post(const su2_bdesc &d, uint32_t s)
     return (i % (uint32_t)d.nevery) == 0u;
 }
 
-template <int MU, int BC, typename LinkT>
+template <int MU, int BC, int ALGO, typename LinkT>
 __device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t h)
 {
-    if (d.algo == 1)
+    if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
         su2_sweep_site<MU, BC, SU2_K_HB, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
     else
         su2_sweep_site<MU, BC, SU2_K_METRO, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
 }
 
-/* OR = 0: link update (heat-bath or Metropolis per lattice); OR = 1:
- * over-relaxation pass number or_pass (lattices with fewer over-relaxation
- * sweeps per tick skip it). */
-template <int MU, int OR, typename LinkT>
+/* OR = 0: link update; ALGO is the algorithm class of the resident lattices
+ * (all heat-bath, all Metropolis, or mixed), so the common homogeneous case
+ * carries one update body.  OR = 1: over-relaxation pass number or_pass
+ * (lattices with fewer over-relaxation sweeps per tick skip it). */
+template <int MU, int OR, int ALGO, typename LinkT>
 __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
-                                                int parity, int or_pass)
+                                                const uint32_t *__restrict__ B, int parity, int or_pass)
 {
-    const uint32_t n = H->n_active, nblk = H->sw_blocks;
-    for (uint32_t vb = blockIdx.x; vb < nblk; vb += gridDim.x) {
-        const su2_bdesc &d = D[find_desc<0>(D, n, vb)];
+    __shared__ uint32_t s_begin[SU2_BATCH_LDS_DESC];
+    const uint32_t n = H->n_active;
+    uint32_t vb0, vend, step;
+    xcd_walk(H->sw_blocks, &vb0, &vend, &step);
+    if (vb0 >= vend) return;
+    const bool lds = stage_begins(B, n, s_begin);
+    for (uint32_t vb = vb0; vb < vend; vb += step) {
+        const su2_bdesc &d = D[find_desc(B, s_begin, lds, n, vb)];
         if (OR && or_pass >= d.or_sweeps) continue;
         const uint32_t h = (vb - d.sw_begin) * blockDim.x + threadIdx.x;
         if (h >= d.g.Vh) continue;
@@ -103,9 +143,9 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
             }
         } else {
             switch (d.bc) {
-            case SU2_K_BC_PER: site_link<MU, SU2_K_BC_PER, LinkT>(d, parity, h); break;
-            case SU2_K_BC_PCC: site_link<MU, SU2_K_BC_PCC, LinkT>(d, parity, h); break;
-            default: site_link<MU, SU2_K_BC_PMC, LinkT>(d, parity, h); break;
+            case SU2_K_BC_PER: site_link<MU, SU2_K_BC_PER, ALGO, LinkT>(d, parity, h); break;
+            case SU2_K_BC_PCC: site_link<MU, SU2_K_BC_PCC, ALGO, LinkT>(d, parity, h); break;
+            default: site_link<MU, SU2_K_BC_PMC, ALGO, LinkT>(d, parity, h); break;
             }
         }
     }
@@ -116,11 +156,18 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
  * of thermalisation also clears the lattice's acceptance counters. */
 template <typename LinkT>
 __global__ __launch_bounds__(256) void k_bnormalize(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
+                                                    const uint32_t *__restrict__ B, uint32_t max_desc,
                                                     const uint32_t *__restrict__ ctr, int post)
 {
-    const uint32_t n = H->n_active, nblk = H->nm_blocks;
-    for (uint32_t vb = blockIdx.x; vb < nblk; vb += gridDim.x) {
-        const su2_bdesc &d = D[find_desc<2>(D, n, vb)];
+    __shared__ uint32_t s_begin[SU2_BATCH_LDS_DESC];
+    const uint32_t n = H->n_active;
+    uint32_t vb0, vend, step;
+    xcd_walk(H->nm_blocks, &vb0, &vend, &step);
+    if (vb0 >= vend) return;
+    const uint32_t *Bk = B + 2u * max_desc;
+    const bool lds = stage_begins(Bk, n, s_begin);
+    for (uint32_t vb = vb0; vb < vend; vb += step) {
+        const su2_bdesc &d = D[find_desc(Bk, s_begin, lds, n, vb)];
         const uint32_t s = ctr[d.slot];
         if (!(post ? normalize_post(d, s) : normalize_pre(d, s))) continue;
         const uint32_t i = (vb - d.nm_begin) * blockDim.x + threadIdx.x;
@@ -157,11 +204,18 @@ __device__ static inline void en_av_block(const su2_bdesc &d, uint32_t b)
 template <typename LinkT>
 __global__ __launch_bounds__(256) void k_ben_av_partial(const su2_bhead *__restrict__ H,
                                                         const su2_bdesc *__restrict__ D,
+                                                        const uint32_t *__restrict__ B, uint32_t max_desc,
                                                         const uint32_t *__restrict__ ctr)
 {
-    const uint32_t n = H->n_active, nblk = H->en_blocks;
-    for (uint32_t vb = blockIdx.x; vb < nblk; vb += gridDim.x) {
-        const su2_bdesc &d = D[find_desc<1>(D, n, vb)];
+    __shared__ uint32_t s_begin[SU2_BATCH_LDS_DESC];
+    const uint32_t n = H->n_active;
+    uint32_t vb0, vend, step;
+    xcd_walk(H->en_blocks, &vb0, &vend, &step);
+    if (vb0 >= vend) return;
+    const uint32_t *Bk = B + max_desc;
+    const bool lds = stage_begins(Bk, n, s_begin);
+    for (uint32_t vb = vb0; vb < vend; vb += step) {
+        const su2_bdesc &d = D[find_desc(Bk, s_begin, lds, n, vb)];
         if (!measure_now(d, ctr[d.slot])) continue;
         const uint32_t b = vb - d.en_begin;
         if (d.bc == SU2_K_BC_PMC)
@@ -215,54 +269,65 @@ __global__ __launch_bounds__(256) void k_btick_end(const su2_bhead *__restrict__
     for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) ctr[D[j].slot] += 1u;
 }
 
-template <int MU, int OR, typename LinkT>
-hipError_t launch_bsweep(const su2_bhead *H, const su2_bdesc *D, int parity, int or_pass, uint32_t grid,
-                         uint32_t block, hipStream_t st)
+template <int MU, int OR, int ALGO, typename LinkT>
+hipError_t launch_bsweep(const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, int parity, int or_pass,
+                         uint32_t grid, uint32_t block, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_bsweep<MU, OR, LinkT>), dim3(grid), dim3(block), 0, st, H, D, parity, or_pass);
+    hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT>), dim3(grid), dim3(block), 0, st, H, D, B, parity, or_pass);
     return hipGetLastError();
 }
 
-template <int OR, typename LinkT>
-hipError_t launch_bsweep_mu(int mu, const su2_bhead *H, const su2_bdesc *D, int parity, int or_pass, uint32_t grid,
-                            uint32_t block, hipStream_t st)
+template <int OR, int ALGO, typename LinkT>
+hipError_t launch_bsweep_mu(int mu, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, int parity,
+                            int or_pass, uint32_t grid, uint32_t block, hipStream_t st)
 {
     switch (mu) {
-    case 0: return launch_bsweep<0, OR, LinkT>(H, D, parity, or_pass, grid, block, st);
-    case 1: return launch_bsweep<1, OR, LinkT>(H, D, parity, or_pass, grid, block, st);
-    default: return launch_bsweep<2, OR, LinkT>(H, D, parity, or_pass, grid, block, st);
+    case 0: return launch_bsweep<0, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st);
+    case 1: return launch_bsweep<1, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st);
+    default: return launch_bsweep<2, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st);
+    }
+}
+
+template <typename LinkT>
+hipError_t launch_bsweep_t(int mu, int over_relax, int algo_class, const su2_bhead *H, const su2_bdesc *D,
+                           const uint32_t *B, int parity, int or_pass, uint32_t grid, uint32_t block, hipStream_t st)
+{
+    if (over_relax) return launch_bsweep_mu<1, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, or_pass, grid, block, st);
+    switch (algo_class) {
+    case SU2_BALGO_HB: return launch_bsweep_mu<0, SU2_BALGO_HB, LinkT>(mu, H, D, B, parity, 0, grid, block, st);
+    case SU2_BALGO_METRO: return launch_bsweep_mu<0, SU2_BALGO_METRO, LinkT>(mu, H, D, B, parity, 0, grid, block, st);
+    default: return launch_bsweep_mu<0, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, 0, grid, block, st);
     }
 }
 
 } // namespace
 
-hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, const su2_bhead *H, const su2_bdesc *D, int parity,
-                             int or_pass, uint32_t grid, uint32_t block, hipStream_t st)
+hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, int algo_class, const su2_bhead *H,
+                             const su2_bdesc *D, const uint32_t *B, uint32_t max_desc, int parity, int or_pass,
+                             uint32_t grid, uint32_t block, hipStream_t st)
 {
-    if (f32)
-        return over_relax ? launch_bsweep_mu<1, su2_l32>(mu, H, D, parity, or_pass, grid, block, st)
-                          : launch_bsweep_mu<0, su2_l32>(mu, H, D, parity, or_pass, grid, block, st);
-    return over_relax ? launch_bsweep_mu<1, su2_l64>(mu, H, D, parity, or_pass, grid, block, st)
-                      : launch_bsweep_mu<0, su2_l64>(mu, H, D, parity, or_pass, grid, block, st);
+    (void)max_desc; /* the sweep range starts are the first array of B */
+    if (f32) return launch_bsweep_t<su2_l32>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st);
+    return launch_bsweep_t<su2_l64>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st);
 }
 
-hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *ctr, int post,
-                                 uint32_t grid, hipStream_t st)
+hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B,
+                                 uint32_t max_desc, const uint32_t *ctr, int post, uint32_t grid, hipStream_t st)
 {
     if (f32)
-        hipLaunchKernelGGL(k_bnormalize<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, ctr, post);
+        hipLaunchKernelGGL(k_bnormalize<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr, post);
     else
-        hipLaunchKernelGGL(k_bnormalize<su2_l64>, dim3(grid), dim3(256), 0, st, H, D, ctr, post);
+        hipLaunchKernelGGL(k_bnormalize<su2_l64>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr, post);
     return hipGetLastError();
 }
 
-hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *ctr, uint32_t grid,
-                             uint32_t max_desc, hipStream_t st)
+hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, uint32_t max_desc,
+                             const uint32_t *ctr, uint32_t grid, hipStream_t st)
 {
     if (f32)
-        hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, ctr);
+        hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr);
     else
-        hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(256), 0, st, H, D, ctr);
+        hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_ben_av_final, dim3(max_desc), dim3(1024), 0, st, H, D, ctr);

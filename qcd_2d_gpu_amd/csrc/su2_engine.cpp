@@ -6,11 +6,15 @@
  * holds each job's field, RNG planes, acceptance counters, en_av partials
  * and measurement series; a compact descriptor table (su2_bdesc) of the
  * resident, unfinished jobs is uploaded from pinned staging whenever the
- * membership changes.  A tick is a fixed sequence of launches whose grids do
- * not depend on the membership, so it can be captured once as a hipGraph
- * and replayed.  Per-job schedules (thermalisation, reunitarisation,
+ * membership changes.  A tick is a sequence of launches whose grids do not
+ * depend on the membership, so it can be captured as a hipGraph and
+ * replayed.  Per-job schedules (thermalisation, reunitarisation,
  * measurement) are evaluated on the device from per-slot sweep counters; the
- * host keeps identical counters to know when a job is done.
+ * host keeps identical counters, to know when a job is done and to leave out
+ * of a tick the launches that no resident job needs in it (reunitarisation,
+ * measurement, over-relaxation passes beyond the largest count, the mixed
+ * heat-bath/Metropolis update body when all jobs use one algorithm).  Each
+ * such tick plan is captured once, on first use, and its graph kept.
  *
  * Server: a thread owning an engine.  Submitted jobs queue; each loop
  * iteration admits queued jobs that fit (arrival order, first that fits),
@@ -110,6 +114,33 @@ struct tune_key {
 std::mutex g_tune_mu;
 std::map<tune_key, std::pair<int, int>> g_tune_cache;
 
+/* Host copies of the device schedule tests (su2_batch.hip) on a job with s
+ * sweeps done. */
+int nevery_of(const su2_job &j) { return j.normalize_every ? j.normalize_every : 100; }
+
+bool h_measure_now(const su2_job &j, int s)
+{
+    if (s < j.n_therm) return false;
+    return (s - j.n_therm) % j.n_cor == j.n_cor - 1;
+}
+
+bool h_normalize_pre(const su2_job &j, int s) { return s < j.n_therm && (s + 1) % nevery_of(j) == 0; }
+
+bool h_normalize_post(const su2_job &j, int s)
+{
+    if (s + 1 == j.n_therm) return true;
+    if (!h_measure_now(j, s)) return false;
+    return ((s - j.n_therm) / j.n_cor + 1) % nevery_of(j) == 0;
+}
+
+/* which launches a tick needs */
+struct tick_plan {
+    bool pre = false, post = false, meas = false;
+    int n_or = 0;
+    int algo = SU2_BALGO_MIXED;
+    int key() const { return (((n_or * 3 + algo) * 2 + (int)pre) * 2 + (int)post) * 2 + (int)meas; }
+};
+
 } // namespace
 
 struct su2_engine {
@@ -122,13 +153,14 @@ struct su2_engine {
     su2_bhead *dH = nullptr;
     su2_bdesc *dD = nullptr;
     uint32_t *dctr = nullptr;
+    uint32_t *dB = nullptr; /* range starts: sweep, en_av, normalise; max_jobs each */
     su2_bhead *hH = nullptr;
     su2_bdesc *hD = nullptr;
+    uint32_t *hB = nullptr;
     hipEvent_t upload_done = nullptr;
     bool upload_pending = false, dirty = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int block = 256, grid = 1024, aux_grid = 1024, ncu = 256;
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs; /* by tick_plan::key */
+    int block = 256, grid = 1024, aux_grid = 1024, en_grid = 2048, ncu = 256;
     int n_active = 0;
 
     bool alloc(size_t n, size_t *off)
@@ -203,6 +235,10 @@ struct su2_engine {
             en += s.lay.en_nb;
             d.nm_begin = nm;
             nm += (6u * s.g.Vh + 255) / 256;
+            const size_t mj = (size_t)cfg.max_jobs;
+            hB[n - 1] = d.sw_begin;
+            hB[mj + n - 1] = d.en_begin;
+            hB[2 * mj + n - 1] = d.nm_begin;
         }
         hH->n_active = n;
         hH->sw_blocks = sw;
@@ -210,26 +246,48 @@ struct su2_engine {
         hH->nm_blocks = nm;
         n_active = (int)n;
         ENG_HIP(hipMemcpyAsync(dH, hH, sizeof(su2_bhead), hipMemcpyHostToDevice, st));
-        if (n) ENG_HIP(hipMemcpyAsync(dD, hD, n * sizeof(su2_bdesc), hipMemcpyHostToDevice, st));
+        if (n) {
+            ENG_HIP(hipMemcpyAsync(dD, hD, n * sizeof(su2_bdesc), hipMemcpyHostToDevice, st));
+            ENG_HIP(hipMemcpyAsync(dB, hB, 3 * sizeof(uint32_t) * cfg.max_jobs, hipMemcpyHostToDevice, st));
+        }
         ENG_HIP(hipEventRecord(upload_done, st));
         upload_pending = true;
         dirty = false;
         return SU2_OK;
     }
 
-    int enqueue_tick()
+    /* the launches the next tick needs, from the host sweep counters */
+    tick_plan plan() const
+    {
+        tick_plan p;
+        bool hb = false, metro = false;
+        for (const auto &s : slots) {
+            if (!s.used || s.done) continue;
+            p.pre |= h_normalize_pre(s.job, s.swept);
+            p.post |= h_normalize_post(s.job, s.swept);
+            p.meas |= h_measure_now(s.job, s.swept);
+            p.n_or = std::max(p.n_or, s.job.or_sweeps);
+            (s.job.algo == SU2_ALGO_HEATBATH ? hb : metro) = true;
+        }
+        p.algo = hb && metro ? SU2_BALGO_MIXED : (hb ? SU2_BALGO_HB : SU2_BALGO_METRO);
+        return p;
+    }
+
+    int enqueue_tick(const tick_plan &p)
     {
         const int f32 = cfg.precision == SU2_PREC_F32;
-        hipError_t e = su2_launch_bnormalize(f32, dH, dD, dctr, 0, aux_grid, st);
+        const uint32_t mj = (uint32_t)cfg.max_jobs;
+        hipError_t e = hipSuccess;
+        if (p.pre) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 0, aux_grid, st);
         for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
             for (int par = 0; par < 2 && e == hipSuccess; ++par)
-                e = su2_launch_bsweep(f32, mu, 0, dH, dD, par, 0, grid, block, st);
-        for (int pass = 0; pass < cfg.max_or && e == hipSuccess; ++pass)
+                e = su2_launch_bsweep(f32, mu, 0, p.algo, dH, dD, dB, mj, par, 0, grid, block, st);
+        for (int pass = 0; pass < p.n_or && e == hipSuccess; ++pass)
             for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
                 for (int par = 0; par < 2 && e == hipSuccess; ++par)
-                    e = su2_launch_bsweep(f32, mu, 1, dH, dD, par, pass, grid, block, st);
-        if (e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dctr, 1, aux_grid, st);
-        if (e == hipSuccess) e = su2_launch_ben_av(f32, dH, dD, dctr, aux_grid, (uint32_t)cfg.max_jobs, st);
+                    e = su2_launch_bsweep(f32, mu, 1, p.algo, dH, dD, dB, mj, par, pass, grid, block, st);
+        if (p.post && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 1, aux_grid, st);
+        if (p.meas && e == hipSuccess) e = su2_launch_ben_av(f32, dH, dD, dB, mj, dctr, en_grid, st);
         if (e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
         if (e != hipSuccess) return su2_set_error(SU2_EHIP, "tick launch: %s", hipGetErrorString(e));
         return SU2_OK;
@@ -237,17 +295,17 @@ struct su2_engine {
 
     void drop_graph()
     {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        for (auto &kv : graphs) {
+            if (kv.second.second) hipGraphExecDestroy(kv.second.second);
+            if (kv.second.first) hipGraphDestroy(kv.second.first);
+        }
+        graphs.clear();
     }
 
-    int capture()
+    int capture(const tick_plan &p, hipGraphExec_t *out)
     {
-        drop_graph();
         ENG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_tick();
+        int rc = enqueue_tick(p);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(st, &g);
         if (rc != SU2_OK) {
@@ -255,8 +313,14 @@ struct su2_engine {
             return rc;
         }
         if (e != hipSuccess) return su2_set_error(SU2_EHIP, "graph capture: %s", hipGetErrorString(e));
-        graph = g;
-        ENG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(g);
+            return su2_set_error(SU2_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+        }
+        graphs[p.key()] = {g, x};
+        *out = x;
         return SU2_OK;
     }
 
@@ -269,14 +333,19 @@ struct su2_engine {
         }
         *any = n_active > 0;
         if (!n_active) return SU2_OK;
+        const tick_plan p = plan();
         if (cfg.use_graph) {
-            if (!exec) {
-                int rc = capture();
+            hipGraphExec_t x = nullptr;
+            auto it = graphs.find(p.key());
+            if (it != graphs.end()) {
+                x = it->second.second;
+            } else {
+                int rc = capture(p, &x);
                 if (rc) return rc;
             }
-            ENG_HIP(hipGraphLaunch(exec, st));
+            ENG_HIP(hipGraphLaunch(x, st));
         } else {
-            int rc = enqueue_tick();
+            int rc = enqueue_tick(p);
             if (rc) return rc;
         }
         for (auto &s : slots)
@@ -318,6 +387,7 @@ int su2_engine_create(const su2_engine_config *cfg, su2_engine **out)
     e->block = cfg->block ? cfg->block : 256;
     e->grid = cfg->grid ? cfg->grid : 4 * e->ncu;
     e->aux_grid = 4 * e->ncu;
+    e->en_grid = 8 * e->ncu; /* en_av fills the chip: 7 workgroups of 256 per CU are resident */
     int rc = SU2_OK;
     auto bail = [&](int code) {
         su2_engine_destroy(e);
@@ -337,8 +407,10 @@ int su2_engine_create(const su2_engine_config *cfg, su2_engine **out)
     if (hipMalloc((void **)&e->dH, sizeof(su2_bhead)) != hipSuccess ||
         hipMalloc((void **)&e->dD, sizeof(su2_bdesc) * e->cfg.max_jobs) != hipSuccess ||
         hipMalloc((void **)&e->dctr, sizeof(uint32_t) * e->cfg.max_jobs) != hipSuccess ||
+        hipMalloc((void **)&e->dB, 3 * sizeof(uint32_t) * e->cfg.max_jobs) != hipSuccess ||
         hipHostMalloc((void **)&e->hH, sizeof(su2_bhead), 0) != hipSuccess ||
         hipHostMalloc((void **)&e->hD, sizeof(su2_bdesc) * e->cfg.max_jobs, 0) != hipSuccess ||
+        hipHostMalloc((void **)&e->hB, 3 * sizeof(uint32_t) * e->cfg.max_jobs, 0) != hipSuccess ||
         hipEventCreateWithFlags(&e->upload_done, hipEventDisableTiming) != hipSuccess)
         return bail(su2_set_error(SU2_ENOMEM, "engine tables"));
     if (hipMemsetAsync(e->dctr, 0, sizeof(uint32_t) * e->cfg.max_jobs, e->st) != hipSuccess) return bail(SU2_EHIP);
@@ -354,6 +426,8 @@ int su2_engine_destroy(su2_engine *e)
     if (e->st) hipStreamSynchronize(e->st);
     e->drop_graph();
     if (e->upload_done) hipEventDestroy(e->upload_done);
+    if (e->hB) hipHostFree(e->hB);
+    if (e->dB) hipFree(e->dB);
     if (e->hD) hipHostFree(e->hD);
     if (e->hH) hipHostFree(e->hH);
     if (e->dctr) hipFree(e->dctr);
@@ -559,6 +633,7 @@ int su2_engine_autotune(su2_engine *e, int *block, int *grid)
     ENG_HIP(hipEventCreate(&a));
     ENG_HIP(hipEventCreate(&b));
     const int f32 = e->cfg.precision == SU2_PREC_F32;
+    const int algo = e->plan().algo;
     const int blocks[3] = {64, 128, 256};
     const int grids[4] = {2 * e->ncu, 4 * e->ncu, 8 * e->ncu, 16 * e->ncu};
     float best = 1e30f;
@@ -575,8 +650,8 @@ int su2_engine_autotune(su2_engine *e, int *block, int *grid)
                 hipEventRecord(a, e->st);
                 for (int mu = 0; mu < 3; ++mu)
                     for (int par = 0; par < 2; ++par)
-                        su2_launch_bsweep(f32, mu, 0, e->dH, e->dD, par, 0, (uint32_t)e->grid, (uint32_t)e->block,
-                                          e->st);
+                        su2_launch_bsweep(f32, mu, 0, algo, e->dH, e->dD, e->dB, (uint32_t)e->cfg.max_jobs, par, 0,
+                                          (uint32_t)e->grid, (uint32_t)e->block, e->st);
                 hipEventRecord(b, e->st);
                 hipEventSynchronize(b);
                 float ms = 0.0f;
