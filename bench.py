@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--batch-jobs", type=int, default=64, help="lattices of the batched ensemble (0: skip)")
     ap.add_argument("--batch-therm", type=int, default=20)
     ap.add_argument("--batch-meas", type=int, default=80)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched-ensemble comparison at --precision and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=100, help="served requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--fp32-meas", type=int, default=1000, help="measurements of the FP32-link check (0: skip)")
@@ -380,6 +382,9 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the single-lattice sweep has no multi-GPU path here", file=sys.stderr)
         sys.exit(2)
+    if args.batched_only:
+        print(json.dumps(batched_bench(args, args.precision)))
+        return
     r = gpu_bench(args)
     V = r["V"]
     ms_step = r["total_ms"] / args.steps

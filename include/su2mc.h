@@ -197,6 +197,73 @@ int su2_abi_version(void);
  * 64, at most 256); 0 restores the built-in default. */
 int su2_set_sweep_block(int threads);
 
+/* ---- handle API: one lattice with its own buffers ----
+ * The same operations as above on an opaque lattice that owns its field,
+ * RNG state, acceptance counters and reduction scratch, all enqueued on its
+ * context's stream.  These are the calls kernel.cu:main makes on its one
+ * lattice (kernel.cu:77-236) with the buffers and the cudaMalloc calls
+ * (kernel.cu:78,82,145,150) folded into the handle.  A context is one
+ * device and one stream; lattices of one context are stream-ordered, and a
+ * context must be used by one host thread at a time.
+ */
+#define SU2_BACKEND_CPU 0 /* not provided: the library has no CPU fallback */
+#define SU2_BACKEND_HIP 1
+
+typedef struct su2_ctx su2_ctx;
+typedef struct su2_lat su2_lat;
+
+/* backend must be SU2_BACKEND_HIP (SU2_BACKEND_CPU returns SU2_EINVAL).
+ * Creates a non-blocking stream on `device`. */
+int su2_ctx_create(int backend, int device, su2_ctx **out);
+int su2_ctx_destroy(su2_ctx *ctx); /* synchronises; the context's lattices must be destroyed first */
+int su2_ctx_stream(su2_ctx *ctx, void **stream);
+int su2_ctx_synchronize(su2_ctx *ctx);
+
+/* Allocates an Lt x Lx x Ly lattice (kernel.cu:58,77-82,145); SU2_EODD_EXTENT
+ * for odd extents (kernel.cu:61-64).  bc = SU2_BC_*, precision = SU2_PREC_*.
+ * The field and RNG are uninitialised until su2_rng_init / su2_init. */
+int su2_lattice_create(su2_ctx *ctx, int Lt, int Lx, int Ly, int bc, int precision, su2_lat **out);
+int su2_lattice_destroy(su2_lat *h);
+/* The lattice's su2_lattice struct (beta, epsilon, updates as last swept),
+ * boundary and precision; NULL skips an output. */
+int su2_lattice_get(const su2_lat *h, su2_lattice *l, int *bc, int *precision);
+/* Raw device buffers of the lattice, for the entry points above; NULL
+ * skips an output. */
+int su2_lattice_buffers(su2_lat *h, void **phi, uint32_t **states, int **accept);
+
+/* setup_kernel (functions/lattice_setup.h:15-21, kernel.cu:79). */
+int su2_rng_init(su2_lat *h, uint64_t seed);
+/* The start of kernel.cu:85-136: SU2_START_COLD id_init2; SU2_START_HOT
+ * rand_init2 (rand_init2_pcc under PCC); SU2_START_HAAR the Haar start.
+ * Also clears the acceptance counters. */
+int su2_init(su2_lat *h, int start);
+/* One sweep (mc_up_per / mc_up_pcc / mc_up_pmc by the lattice's boundary,
+ * kernel.cu:167,195): algo SU2_ALGO_*, update SU2_UPDATE_*; beta, eps and
+ * hits as the reference's betal, epsilonl and upd. */
+int su2_sweep(su2_lat *h, int algo, int update, float beta, float eps, int hits);
+/* normalize (functions/lattice_setup.h:202-207) on the handle's field
+ * (su2_normalize is the raw-pointer form). */
+int su2_renormalize(su2_lat *h);
+/* en_av<PMC> (functions/observables.h:99-141) with PMC set by the lattice's
+ * boundary; synchronous: *E and *P receive the energy and the average
+ * plaquette (host pointers).  Deterministic, like su2_en_av. */
+int su2_measure(su2_lat *h, double *E, double *P);
+/* Acceptance counters (kernel.cu:144-145,190,225): *accepted = sum over
+ * sites, *attempts = V x (3 x hits for each link sweep, 3 for each
+ * over-relaxation sweep) since su2_init or su2_accept_reset, the
+ * denominator of kernel.cu:225,246 (no PCC correction); synchronous. */
+int su2_accept_read(su2_lat *h, int64_t *accepted, int64_t *attempts);
+int su2_accept_reset(su2_lat *h); /* cudaMemset of kernel.cu:190 */
+/* Field in the reference layout double[12 V] on the host (the Config_ file
+ * layout, functions/output.h:85, functions/lattice_setup.h:83-100);
+ * synchronous. */
+int su2_field_download(su2_lat *h, double *host_phi_ref);
+int su2_field_upload(su2_lat *h, const double *host_phi_ref);
+/* RNG state uint32[6 * V/2] on the host (six planes of V/2); synchronous.
+ * Extension: the reference cannot save its RNG state. */
+int su2_rng_download(su2_lat *h, uint32_t *host_states);
+int su2_rng_upload(su2_lat *h, const uint32_t *host_states);
+
 /* ---- batched lattices: one launch per (direction, parity) for many ----
  *
  * Extension (no counterpart in the reference, whose driver runs one lattice

@@ -35,6 +35,8 @@ START_COLD = 1
 START_HAAR = 2
 PREC_F64 = 0
 PREC_F32 = 1
+BACKEND_CPU = 0
+BACKEND_HIP = 1
 
 
 class Su2Lattice(C.Structure):
@@ -119,6 +121,25 @@ PROTOTYPES = {
     "su2_last_error": (C.c_char_p, []),
     "su2_abi_version": (_I, []),
     "su2_set_sweep_block": (_I, [_I]),
+    "su2_ctx_create": (_I, [_I, _I, C.POINTER(C.c_void_p)]),
+    "su2_ctx_destroy": (_I, [_P]),
+    "su2_ctx_stream": (_I, [_P, C.POINTER(C.c_void_p)]),
+    "su2_ctx_synchronize": (_I, [_P]),
+    "su2_lattice_create": (_I, [_P, _I, _I, _I, _I, _I, C.POINTER(C.c_void_p)]),
+    "su2_lattice_destroy": (_I, [_P]),
+    "su2_lattice_get": (_I, [_P, _L, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "su2_lattice_buffers": (_I, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "su2_rng_init": (_I, [_P, C.c_uint64]),
+    "su2_init": (_I, [_P, _I]),
+    "su2_sweep": (_I, [_P, _I, _I, C.c_float, C.c_float, _I]),
+    "su2_renormalize": (_I, [_P]),
+    "su2_measure": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "su2_accept_read": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "su2_accept_reset": (_I, [_P]),
+    "su2_field_download": (_I, [_P, _P]),
+    "su2_field_upload": (_I, [_P, _P]),
+    "su2_rng_download": (_I, [_P, _P]),
+    "su2_rng_upload": (_I, [_P, _P]),
     "su2_run_config_default": (None, [C.POINTER(Su2RunConfig)]),
     "su2_run": (_I, [C.POINTER(Su2RunConfig), C.POINTER(Su2RunResult)]),
 }

@@ -94,3 +94,21 @@ def test_no_torch_in_product_package():
             src = open(os.path.join(pkg, fn)).read()
             assert "import torch" not in src
             assert "oracle" not in re.sub(r"#.*", "", src).lower().replace("oracle/", ""), fn
+
+
+def test_handle_api_rejects_cpu_backend_and_null_handles():
+    """No CPU fallback: the CPU backend is refused before any device call."""
+    lib = q.load_library()
+    c = C.c_void_p(123)
+    assert lib.su2_ctx_create(L.BACKEND_CPU, 0, C.byref(c)) == L.SU2_EINVAL
+    assert c.value is None
+    assert b"no CPU backend" in lib.su2_last_error()
+    assert lib.su2_ctx_create(5, 0, C.byref(c)) == L.SU2_EINVAL
+    h = C.c_void_p()
+    assert lib.su2_lattice_create(None, 4, 4, 4, 0, 0, C.byref(h)) == L.SU2_EINVAL
+    for name in ["su2_renormalize", "su2_accept_reset"]:
+        assert getattr(lib, name)(None) == L.SU2_EINVAL
+    assert lib.su2_rng_init(None, 1) == L.SU2_EINVAL
+    assert lib.su2_sweep(None, 1, 0, 1.0, 0.1, 1) == L.SU2_EINVAL
+    assert lib.su2_lattice_destroy(None) == 0
+    assert lib.su2_ctx_destroy(None) == 0
