@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 BYTES_HB = 240.0  # per heat-bath link update, FP64 links
 BYTES_OR = 224.0  # per over-relaxation link update, FP64 links
 
@@ -186,9 +187,17 @@ def cpu_baseline(args, V_gpu):
         return None
     sec = o.time_sweeps(0, 1, 1, steps)
     ups = 2 * 3 * o.V * steps / sec
-    return {"value": ups, "unit": "link updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps (heat-bath + over-relaxation sweep) of the same {L}^3 lattice, "
-                      f"{sec:.1f} s, OpenMP C restatement of the reference"}
+    out = {"value": ups, "unit": "link updates/s", "cores": threads, "kind": "port",
+           "sample": f"{steps} steps (heat-bath + over-relaxation sweep) of the same {L}^3 lattice, "
+                     f"{sec:.1f} s, OpenMP C restatement of the reference"}
+    if threads > 1 and args.cpu_steps < 0:
+        # the same restatement on one thread, over a step or two
+        O.set_threads(1)
+        s1 = max(1, min(2, int(steps / threads)))
+        sec1 = o.time_sweeps(0, 1, 1, s1)
+        out["one_thread"] = {"value": 2 * 3 * o.V * s1 / sec1, "steps": s1, "seconds": sec1}
+        O.set_threads(threads)
+    return out
 
 
 def large_lattice_bench(args):
@@ -363,6 +372,18 @@ def fp32_check(args):
     return out
 
 
+def load_profile_value(fname, key):
+    """The number on the line "key: N" of a committed rocprofv3 summary
+    under profiles/, or None."""
+    p = os.path.join(ROOT, "profiles", fname)
+    if os.path.exists(p):
+        with open(p) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
+    return None
+
+
 def load_traffic():
     """HBM bytes per heat-bath launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present:
@@ -374,6 +395,22 @@ def load_traffic():
                 if line.startswith("bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
     return None
+
+
+def fp64_rate(r, updates):
+    """FP64 flop rate of the heat-bath and over-relaxation launches: flops
+    per update counted by rocprofv3 (SQ_INSTS_VALU_FLOPS_FP64 + _TRANS,
+    profiles/sweep_128cube_flops.txt) x updates / the launch time measured
+    here, against the 78.6 TFLOP/s FP64 vector peak."""
+    hb = load_profile_value("sweep_128cube_flops.txt", "hb_fp64_flops_per_update")
+    orf = load_profile_value("sweep_128cube_flops.txt", "or_fp64_flops_per_update")
+    if hb is None or orf is None:
+        return None
+    hb_tf = hb * updates / (r["hb_ms"] * 1e-3) / 1e12
+    or_tf = orf * updates / (r["or_ms"] * 1e-3) / 1e12
+    return {"heatbath_tflops": hb_tf, "overrelax_tflops": or_tf, "peak_tflops": FP64_PEAK_TFLOPS,
+            "heatbath_frac": hb_tf / FP64_PEAK_TFLOPS, "flops_per_update_heatbath": hb,
+            "flops_per_update_overrelax": orf}
 
 
 def main():
@@ -429,7 +466,9 @@ def main():
                      "kernel": "heat-bath sweep launch (one direction x parity)",
                      "bytes_per_update": BYTES_HB, "updates_per_launch": per_launch_updates,
                      "avg_launch_us": r["hb_ms"] * 1e3},
-        "overrelax_launch": {"avg_launch_us": r["or_ms"] * 1e3, "achieved_gbs": or_gbs},
+        "overrelax_launch": {"avg_launch_us": r["or_ms"] * 1e3, "achieved_gbs": or_gbs,
+                             "updates_per_s_kernel": per_launch_updates / (r["or_ms"] * 1e-3)},
+        "fp64_rate": fp64_rate(r, per_launch_updates),
         "step_algorithmic_gbs": step_bytes / (ms_step * 1e-3) / 1e9,
         "heatbath_updates_per_s_kernel": per_launch_updates / (r["hb_ms"] * 1e-3),
         "observables": {"energy": r["E"], "plaquette": r["P"]},

@@ -119,3 +119,33 @@ def test_cli(gpu, tmp_path):
     assert (tmp_path / "Check_8_8_8_B_3.00_per_1.txt").exists()
     bad = subprocess.run([exe, "--Lt", "7"], capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "even" in bad.stderr
+
+
+def test_fp32_links_agree_with_fp64_within_binned_errors(gpu):
+    """The boundary + Metropolis + precision configuration (32x32x48, PCC and
+    PMC, eps 0.5, 4 hits, beta 4, hot start): FP32 link storage gives the
+    FP64 average plaquette within 3 combined binned errors (20 bins)."""
+    q = gpu
+    for bc in (q.BC_PCC, q.BC_PMC):
+        print(f"[driver] FP32 vs FP64 bc={bc}", flush=True)
+        P = {}
+        for prec in (q.PREC_F64, q.PREC_F32):
+            r = q.run(Lt=32, Lx=32, Ly=48, beta=4.0, boundary=bc, algo=q.ALGO_METROPOLIS, init=q.START_HOT,
+                      epsilon=0.5, upd=4, N_term=200, N=600, seed=4321, precision=prec)
+            P[prec] = q.stats.binned_error(r.plaquette, 20)
+        (m64, e64), (m32, e32) = P[q.PREC_F64], P[q.PREC_F32]
+        assert abs(m32 - m64) < 3 * np.hypot(e64, e32), (bc, m64, e64, m32, e32)
+
+
+def test_strong_coupling_on_gpu(gpu):
+    """beta = 0.5: <P> = u + 4 u^5 with u = I2(beta)/I1(beta) (strong-coupling
+    expansion of the 3-D SU(2) plaquette), within 4 binned errors."""
+    from scipy.special import iv
+
+    q = gpu
+    print("[driver] strong coupling 16^3 beta 0.5", flush=True)
+    r = q.run(Lt=16, Lx=16, Ly=16, beta=0.5, boundary=0, algo=q.ALGO_HEATBATH, init=q.START_HOT, N_term=100,
+              N=1000, seed=11)
+    u = iv(2, 0.5) / iv(1, 0.5)
+    m, e = q.stats.binned_error(r.plaquette, 20)
+    assert abs(m - (u + 4 * u ** 5)) < 4 * e + 2e-4, (m, e, u + 4 * u ** 5)
