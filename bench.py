@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--batch-meas", type=int, default=80)
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-ensemble comparison at --precision and print its JSON")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving stream and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=100, help="served requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--fp32-meas", type=int, default=1000, help="measurements of the FP32-link check (0: skip)")
@@ -421,6 +422,9 @@ def main():
         sys.exit(2)
     if args.batched_only:
         print(json.dumps(batched_bench(args, args.precision)))
+        return
+    if args.serving_only:
+        print(json.dumps(serving_bench(args)))
         return
     r = gpu_bench(args)
     V = r["V"]

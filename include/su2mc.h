@@ -85,7 +85,12 @@ typedef struct su2_lattice {
 } su2_lattice;
 
 /* Fills *l for an Lt x Lx x Ly lattice (kernel.cu:58).  Returns
- * SU2_EODD_EXTENT when an extent is odd or below 2 (kernel.cu:61-64). */
+ * SU2_EODD_EXTENT when an extent is odd or below 2 (kernel.cu:61-64) and
+ * SU2_EINVAL above SU2_MAX_LATTICE_SITES sites (3V + padding must fit the
+ * kernels' 32-bit link indices and a dispatch's 2^32 work-items; this is
+ * 160 GB of FP64 device state). */
+#define SU2_MAX_LATTICE_SITES 1431633920ll
+
 int su2_lattice_init(su2_lattice *l, int Lt, int Lx, int Ly, float beta, float epsilon, int updates);
 
 /* Buffer sizes in bytes. */
@@ -274,8 +279,9 @@ int su2_rng_upload(su2_lat *h, const uint32_t *host_states);
  * and gives bit for bit the results it gives alone.  A tick advances every
  * resident, unfinished job by one update sweep (+ its over-relaxation
  * sweeps), with its reunitarisation and measurement when due.  Ticks can be
- * replayed from one captured hipGraph (use_graph); jobs may join or leave
- * between ticks.
+ * replayed from captured hipGraphs (use_graph); jobs may join or leave
+ * between ticks.  Calls on one engine must come from one host thread at a
+ * time (the engine of a server belongs to the server's thread).
  */
 typedef struct su2_job {
     int Lt, Lx, Ly;

@@ -227,16 +227,22 @@ __global__ __launch_bounds__(256) void k_ben_av_partial(const su2_bhead *__restr
 }
 
 /* en_av pass 2: workgroup j (1024 threads) adds lattice j's partials in the
- * single-lattice order and accumulates sum / V into its measurement slot. */
+ * single-lattice order and accumulates sum / V into its measurement slot.
+ * It is the last launch of a measuring tick, so it also ends the tick:
+ * every lattice's sweep counter is read once and then advanced by one. */
 __global__ __launch_bounds__(1024) void k_ben_av_final(const su2_bhead *__restrict__ H,
                                                        const su2_bdesc *__restrict__ D,
-                                                       const uint32_t *__restrict__ ctr)
+                                                       uint32_t *__restrict__ ctr)
 {
     const uint32_t n = H->n_active;
     for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
         const su2_bdesc &d = D[j];
         const uint32_t s = ctr[d.slot];
-        if (!measure_now(d, s)) continue;
+        __syncthreads(); /* every wave has read the counter before lane 0 advances it */
+        if (!measure_now(d, s)) {
+            if (threadIdx.x == 0) ctr[d.slot] = s + 1u;
+            continue;
+        }
         const uint32_t m = (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor;
         double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
         for (uint32_t i = threadIdx.x; i < d.en_nb; i += 4 * blockDim.x) {
@@ -256,12 +262,14 @@ __global__ __launch_bounds__(1024) void k_ben_av_final(const su2_bhead *__restri
             const double invV = 1.0 / (2.0 * (double)d.g.Vh);
             d.resE[m] += te * invV;
             d.resP[m] += tp * invV;
+            ctr[d.slot] = s + 1u;
         }
         __syncthreads();
     }
 }
 
-/* End of a tick: every resident lattice has done one more sweep. */
+/* End of a tick without measurement: every resident lattice has done one
+ * more sweep. */
 __global__ __launch_bounds__(256) void k_btick_end(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
                                                    uint32_t *__restrict__ ctr)
 {
@@ -322,7 +330,7 @@ hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D
 }
 
 hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, uint32_t max_desc,
-                             const uint32_t *ctr, uint32_t grid, hipStream_t st)
+                             uint32_t *ctr, uint32_t grid, hipStream_t st)
 {
     if (f32)
         hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr);

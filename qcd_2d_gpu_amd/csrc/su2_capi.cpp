@@ -49,7 +49,7 @@ int check_lattice(const su2_lattice *l)
     if (l->t < 2 || l->x < 2 || l->y < 2 || (l->t & 1) || (l->x & 1) || (l->y & 1))
         return fail(SU2_EODD_EXTENT, "Lt, Lx and Ly must be even and >= 2 (got %d %d %d)", l->t, l->x, l->y);
     long long V = (long long)l->t * l->x * l->y;
-    if (V > (1ll << 31) / 3 * 2) return fail(SU2_EINVAL, "lattice too large (V = %lld)", V);
+    if (V > SU2_MAX_SITES) return fail(SU2_EINVAL, "lattice too large (V = %lld > %lld)", V, SU2_MAX_SITES);
     if (l->V != V) return fail(SU2_EINVAL, "lattice V = %d does not equal Lt*Lx*Ly = %lld", l->V, V);
     return SU2_OK;
 }

@@ -83,7 +83,7 @@ int check_job(const su2_job *j, int max_or)
     if (!j) return su2_set_error(SU2_EINVAL, "job is NULL");
     if (j->Lt < 2 || j->Lx < 2 || j->Ly < 2 || (j->Lt & 1) || (j->Lx & 1) || (j->Ly & 1))
         return su2_set_error(SU2_EODD_EXTENT, "job extents must be even and >= 2");
-    if ((long long)j->Lt * j->Lx * j->Ly > (1ll << 31) / 3 * 2) return su2_set_error(SU2_EINVAL, "job too large");
+    if ((long long)j->Lt * j->Lx * j->Ly > SU2_MAX_SITES) return su2_set_error(SU2_EINVAL, "job too large");
     if (j->n_therm < 1 || j->n_meas < 1 || j->n_cor < 1 || j->or_sweeps < 0 || j->or_sweeps > max_or ||
         j->normalize_every < 0)
         return su2_set_error(SU2_EINVAL, "job schedule out of range (n_therm, n_meas, n_cor >= 1; or_sweeps <= %d)",
@@ -288,7 +288,7 @@ struct su2_engine {
                     e = su2_launch_bsweep(f32, mu, 1, p.algo, dH, dD, dB, mj, par, pass, grid, block, st);
         if (p.post && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 1, aux_grid, st);
         if (p.meas && e == hipSuccess) e = su2_launch_ben_av(f32, dH, dD, dB, mj, dctr, en_grid, st);
-        if (e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
+        if (!p.meas && e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
         if (e != hipSuccess) return su2_set_error(SU2_EHIP, "tick launch: %s", hipGetErrorString(e));
         return SU2_OK;
     }
@@ -622,53 +622,65 @@ int su2_engine_autotune(su2_engine *e, int *block, int *grid)
         }
     }
     /* Trials run real update sweeps, so the resident state is saved first and
-     * restored afterwards: tuning leaves every job's results unchanged. */
+     * restored afterwards, also when a trial fails: tuning leaves every job's
+     * results unchanged. */
     size_t hi = 0;
     for (auto &s : e->slots)
         if (s.used) hi = std::max(hi, s.off + s.lay.total);
     char *backup = nullptr;
     ENG_HIP(hipMalloc((void **)&backup, hi));
-    ENG_HIP(hipMemcpyAsync(backup, e->arena, hi, hipMemcpyDeviceToDevice, e->st));
-    hipEvent_t a, b;
-    ENG_HIP(hipEventCreate(&a));
-    ENG_HIP(hipEventCreate(&b));
+    hipEvent_t a = nullptr, b = nullptr;
+    const int keep_block = e->block, keep_grid = e->grid;
+    int bb = keep_block, bg = keep_grid;
+    hipError_t he = hipMemcpyAsync(backup, e->arena, hi, hipMemcpyDeviceToDevice, e->st);
+    if (he == hipSuccess) he = hipEventCreate(&a);
+    if (he == hipSuccess) he = hipEventCreate(&b);
+    int rc = he == hipSuccess ? SU2_OK : su2_set_error(SU2_EHIP, "autotune set-up: %s", hipGetErrorString(he));
     const int f32 = e->cfg.precision == SU2_PREC_F32;
     const int algo = e->plan().algo;
     const int blocks[3] = {64, 128, 256};
     const int grids[4] = {2 * e->ncu, 4 * e->ncu, 8 * e->ncu, 16 * e->ncu};
     float best = 1e30f;
-    int bb = e->block, bg = e->grid;
-    for (int bi = 0; bi < 3; ++bi)
-        for (int gi = 0; gi < 4; ++gi) {
+    for (int bi = 0; bi < 3 && rc == SU2_OK; ++bi)
+        for (int gi = 0; gi < 4 && rc == SU2_OK; ++gi) {
             e->block = blocks[bi];
             e->grid = grids[gi];
             e->dirty = true;
-            int rc = e->rebuild();
-            if (rc) return rc;
+            rc = e->rebuild();
             float tot = 0.0f;
-            for (int rep = 0; rep < 3; ++rep) {
-                hipEventRecord(a, e->st);
-                for (int mu = 0; mu < 3; ++mu)
-                    for (int par = 0; par < 2; ++par)
-                        su2_launch_bsweep(f32, mu, 0, algo, e->dH, e->dD, e->dB, (uint32_t)e->cfg.max_jobs, par, 0,
-                                          (uint32_t)e->grid, (uint32_t)e->block, e->st);
-                hipEventRecord(b, e->st);
-                hipEventSynchronize(b);
+            for (int rep = 0; rep < 3 && rc == SU2_OK; ++rep) {
+                he = hipEventRecord(a, e->st);
+                for (int mu = 0; mu < 3 && he == hipSuccess; ++mu)
+                    for (int par = 0; par < 2 && he == hipSuccess; ++par)
+                        he = su2_launch_bsweep(f32, mu, 0, algo, e->dH, e->dD, e->dB, (uint32_t)e->cfg.max_jobs, par,
+                                               0, (uint32_t)e->grid, (uint32_t)e->block, e->st);
+                if (he == hipSuccess) he = hipEventRecord(b, e->st);
+                if (he == hipSuccess) he = hipEventSynchronize(b);
                 float ms = 0.0f;
-                hipEventElapsedTime(&ms, a, b);
+                if (he == hipSuccess) he = hipEventElapsedTime(&ms, a, b);
+                if (he != hipSuccess) rc = su2_set_error(SU2_EHIP, "autotune trial: %s", hipGetErrorString(he));
                 if (rep) tot += ms;
             }
-            if (tot < best) {
+            if (rc == SU2_OK && tot < best) {
                 best = tot;
                 bb = e->block;
                 bg = e->grid;
             }
         }
-    ENG_HIP(hipMemcpyAsync(e->arena, backup, hi, hipMemcpyDeviceToDevice, e->st));
-    ENG_HIP(hipStreamSynchronize(e->st));
+    /* restore the resident state whatever happened above */
+    he = hipMemcpyAsync(e->arena, backup, hi, hipMemcpyDeviceToDevice, e->st);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->st);
     hipFree(backup);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    if (he != hipSuccess && rc == SU2_OK)
+        rc = su2_set_error(SU2_EHIP, "autotune restore: %s", hipGetErrorString(he));
+    if (rc != SU2_OK) {
+        e->block = keep_block;
+        e->grid = keep_grid;
+        e->dirty = true;
+        return rc;
+    }
     e->block = bb;
     e->grid = bg;
     e->dirty = true;
@@ -771,12 +783,25 @@ struct su2_server {
         publish(std::move(f));
     }
 
+    /* records a fatal error: the loop stops and drain/poll report it */
+    void fail(int rc)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (error == SU2_OK) error = rc;
+        stop = true;
+        idle_cv.notify_all();
+    }
+
     void loop()
     {
-        hipSetDevice(e->cfg.device);
         hipEvent_t ev[2] = {nullptr, nullptr};
-        hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
-        hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+        hipError_t he = hipSetDevice(e->cfg.device);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+        if (he != hipSuccess) {
+            fail(su2_set_error(SU2_EHIP, "server set-up: %s", hipGetErrorString(he)));
+            return;
+        }
         int cur = 0;
         bool prev_valid = false;
         std::vector<running_job> prev_done;
@@ -821,14 +846,12 @@ struct su2_server {
             bool ticked = false;
             if (!running.empty()) {
                 int rc = su2_engine_tick(e, 1);
+                if (rc == SU2_OK && (he = hipEventRecord(ev[cur], e->st)) != hipSuccess)
+                    rc = su2_set_error(SU2_EHIP, "server tick event: %s", hipGetErrorString(he));
                 if (rc != SU2_OK) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    error = rc;
-                    stop = true;
-                    idle_cv.notify_all();
+                    fail(rc);
                     break;
                 }
-                hipEventRecord(ev[cur], e->st);
                 ticked = true;
                 for (auto it = running.begin(); it != running.end();) {
                     int d = 0, t = 0;
@@ -843,7 +866,10 @@ struct su2_server {
             }
             /* retire the jobs the previous tick finished while this one runs */
             if (prev_valid) {
-                hipEventSynchronize(ev[cur ^ 1]);
+                if ((he = hipEventSynchronize(ev[cur ^ 1])) != hipSuccess) {
+                    fail(su2_set_error(SU2_EHIP, "server tick wait: %s", hipGetErrorString(he)));
+                    break;
+                }
                 for (auto &r : prev_done) retire(r);
                 prev_done.clear();
                 prev_valid = false;
@@ -854,9 +880,11 @@ struct su2_server {
                 cur ^= 1;
             }
         }
-        if (prev_valid) {
-            hipEventSynchronize(ev[cur ^ 1]);
-            for (auto &r : prev_done) retire(r);
+        if (prev_valid && error == SU2_OK) {
+            if ((he = hipEventSynchronize(ev[cur ^ 1])) == hipSuccess)
+                for (auto &r : prev_done) retire(r);
+            else
+                fail(su2_set_error(SU2_EHIP, "server tick wait: %s", hipGetErrorString(he)));
         }
         hipEventDestroy(ev[0]);
         hipEventDestroy(ev[1]);

@@ -24,6 +24,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+/* Largest lattice, in sites.  Link indices (p * 3 + mu) * Vh + h < 3V are
+ * 32-bit in the kernels, and every launch's thread count padded to whole
+ * blocks (at most 3V + 2047) must stay below 2^32, the dispatch limit:
+ * 3V <= 2^32 - 2^16.  (1.43 G sites = 160 GB of FP64 links, RNG state and
+ * acceptance counters, in 288 GB of HBM.) */
+#define SU2_MAX_SITES 1431633920ll /* = SU2_MAX_LATTICE_SITES of su2mc.h */
+
 enum { SU2_DIR_Y = 0, SU2_DIR_X = 1, SU2_DIR_T = 2 };
 enum { SU2_K_BC_PER = 0, SU2_K_BC_PCC = 1, SU2_K_BC_PMC = 2 };
 enum { SU2_K_METRO = 0, SU2_K_HB = 1, SU2_K_OR = 2 };

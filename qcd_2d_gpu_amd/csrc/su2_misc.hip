@@ -69,9 +69,8 @@ __global__ __launch_bounds__(256) void k_init(LinkT *__restrict__ U, uint32_t *_
 /* normalize (functions/lattice_setup.h:202-207): every link a <- a / |a|. */
 template <typename LinkT> __global__ __launch_bounds__(256) void k_normalize(LinkT *__restrict__ U, uint32_t nlinks)
 {
-    const uint32_t i = gtid();
-    if (i >= nlinks) return;
-    su2_link_io<LinkT>::store(U + i, su2_q_normalize(su2_link_io<LinkT>::load(U + i)));
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nlinks; i += (size_t)gridDim.x * blockDim.x)
+        su2_link_io<LinkT>::store(U + i, su2_q_normalize(su2_link_io<LinkT>::load(U + i)));
 }
 
 /* Pass 1 of en_av, one site per thread.  Logical block b covers the sites
@@ -133,25 +132,29 @@ __global__ __launch_bounds__(1024) void k_en_av_final(const double *__restrict__
 template <typename LinkT>
 __global__ __launch_bounds__(256) void k_to_ref(const LinkT *__restrict__ U, double *__restrict__ phi, uint32_t Vh)
 {
-    const uint32_t i = gtid();
-    if (i >= 6u * Vh) return;
-    const uint32_t pm = i / Vh, h = i - pm * Vh, p = pm / 3u, mu = pm - 3u * p;
-    su2_q l = su2_link_io<LinkT>::load(U + i);
-    double *d = phi + ((size_t)(p * Vh + h) * 3 + mu) * 4;
-    d[0] = l.a0;
-    d[1] = l.a1;
-    d[2] = l.a2;
-    d[3] = l.a3;
+    const size_t n = 6 * (size_t)Vh;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)j; /* < 3V < 2^32 */
+        const uint32_t pm = i / Vh, h = i - pm * Vh, p = pm / 3u, mu = pm - 3u * p;
+        su2_q l = su2_link_io<LinkT>::load(U + i);
+        double *d = phi + ((size_t)(p * Vh + h) * 3 + mu) * 4;
+        d[0] = l.a0;
+        d[1] = l.a1;
+        d[2] = l.a2;
+        d[3] = l.a3;
+    }
 }
 
 template <typename LinkT>
 __global__ __launch_bounds__(256) void k_from_ref(const double *__restrict__ phi, LinkT *__restrict__ U, uint32_t Vh)
 {
-    const uint32_t i = gtid();
-    if (i >= 6u * Vh) return;
-    const uint32_t pm = i / Vh, h = i - pm * Vh, p = pm / 3u, mu = pm - 3u * p;
-    const double *d = phi + ((size_t)(p * Vh + h) * 3 + mu) * 4;
-    su2_link_io<LinkT>::store(U + i, su2_q_make(d[0], d[1], d[2], d[3]));
+    const size_t n = 6 * (size_t)Vh;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)j; /* < 3V < 2^32 */
+        const uint32_t pm = i / Vh, h = i - pm * Vh, p = pm / 3u, mu = pm - 3u * p;
+        const double *d = phi + ((size_t)(p * Vh + h) * 3 + mu) * 4;
+        su2_link_io<LinkT>::store(U + i, su2_q_make(d[0], d[1], d[2], d[3]));
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int *__restrict__ p, int v, size_t n)
@@ -217,10 +220,18 @@ hipError_t su2_launch_init(int start, int f32, void *U, uint32_t *rng, const su2
     return f32 ? launch_init_t<su2_l32>(start, U, rng, g, st) : launch_init_t<su2_l64>(start, U, rng, g, st);
 }
 
+/* Blocks of 256 for a grid-stride walk over n items: one item per thread
+ * up to 2^20 blocks (about 1 100 per CU), then each thread takes several. */
+static uint32_t stride_blocks(size_t n)
+{
+    const size_t nb = (n + 255) / 256;
+    return (uint32_t)(nb < ((size_t)1 << 20) ? nb : ((size_t)1 << 20));
+}
+
 hipError_t su2_launch_normalize(int f32, void *U, const su2_kgeom &g, hipStream_t st)
 {
-    const uint32_t n = 6u * g.Vh;
-    const uint32_t nb = (n + 255) / 256;
+    const uint32_t n = 6u * g.Vh; /* 3V < 2^32 */
+    const uint32_t nb = stride_blocks(n);
     if (f32)
         hipLaunchKernelGGL(k_normalize<su2_l32>, dim3(nb), dim3(256), 0, st, (su2_l32 *)U, n);
     else
@@ -259,7 +270,7 @@ hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g,
 
 hipError_t su2_launch_to_ref(int f32, const void *U, double *phi_ref, const su2_kgeom &g, hipStream_t st)
 {
-    const uint32_t nb = (6u * g.Vh + 255) / 256;
+    const uint32_t nb = stride_blocks(6 * (size_t)g.Vh);
     if (f32)
         hipLaunchKernelGGL(k_to_ref<su2_l32>, dim3(nb), dim3(256), 0, st, (const su2_l32 *)U, phi_ref, g.Vh);
     else
@@ -269,7 +280,7 @@ hipError_t su2_launch_to_ref(int f32, const void *U, double *phi_ref, const su2_
 
 hipError_t su2_launch_from_ref(int f32, const double *phi_ref, void *U, const su2_kgeom &g, hipStream_t st)
 {
-    const uint32_t nb = (6u * g.Vh + 255) / 256;
+    const uint32_t nb = stride_blocks(6 * (size_t)g.Vh);
     if (f32)
         hipLaunchKernelGGL(k_from_ref<su2_l32>, dim3(nb), dim3(256), 0, st, phi_ref, (su2_l32 *)U, g.Vh);
     else

@@ -112,3 +112,19 @@ def test_handle_api_rejects_cpu_backend_and_null_handles():
     assert lib.su2_sweep(None, 1, 0, 1.0, 0.1, 1) == L.SU2_EINVAL
     assert lib.su2_lattice_destroy(None) == 0
     assert lib.su2_ctx_destroy(None) == 0
+
+
+def test_size_limit():
+    """3V and every padded launch must stay below 2^32: the largest lattice
+    has SU2_MAX_LATTICE_SITES = 1 431 633 920 sites."""
+    lib = q.load_library()
+    l = L.Su2Lattice()
+    # 1 431 633 920 = 2 * 2 * 357 908 480 is accepted, 8 more sites are not
+    assert lib.su2_lattice_init(C.byref(l), 2, 2, 357908480, 1.0, 0.1, 1) == 0
+    assert l.V == 1431633920
+    assert lib.su2_lattice_init(C.byref(l), 2, 2, 357908482, 1.0, 0.1, 1) == L.SU2_EINVAL
+    assert b"too large" in lib.su2_last_error()
+    assert lib.su2_lattice_init(C.byref(l), 1126, 1126, 1126, 1.0, 0.1, 1) == 0
+    assert lib.su2_lattice_init(C.byref(l), 1128, 1128, 1128, 1.0, 0.1, 1) == L.SU2_EINVAL
+    big = q.Job(1128, 1128, 1128, 3.0)
+    assert lib.su2_job_bytes(C.byref(big.c()), 0) == 0
