@@ -49,7 +49,7 @@ hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D
 /* en_av of the lattices that measure this tick; its final pass also advances
  * every sweep counter, so a measuring tick needs no su2_launch_btick_end. */
 hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, uint32_t max_desc,
-                             uint32_t *ctr, uint32_t grid, hipStream_t st);
+                             uint32_t *ctr, uint32_t grid, uint32_t final_grid, hipStream_t st);
 hipError_t su2_launch_btick_end(const su2_bhead *H, const su2_bdesc *D, uint32_t *ctr, hipStream_t st);
 uint32_t su2_en_av_logical_blocks(const su2_kgeom &g);
 

@@ -330,7 +330,7 @@ hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D
 }
 
 hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, uint32_t max_desc,
-                             uint32_t *ctr, uint32_t grid, hipStream_t st)
+                             uint32_t *ctr, uint32_t grid, uint32_t final_grid, hipStream_t st)
 {
     if (f32)
         hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr);
@@ -338,7 +338,7 @@ hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, co
         hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ben_av_final, dim3(max_desc), dim3(1024), 0, st, H, D, ctr);
+    hipLaunchKernelGGL(k_ben_av_final, dim3(final_grid), dim3(1024), 0, st, H, D, ctr);
     return hipGetLastError();
 }
 

@@ -133,13 +133,35 @@ bool h_normalize_post(const su2_job &j, int s)
     return ((s - j.n_therm) / j.n_cor + 1) % nevery_of(j) == 0;
 }
 
-/* which launches a tick needs */
+/* which launches a tick needs, and their grids */
 struct tick_plan {
     bool pre = false, post = false, meas = false;
     int n_or = 0;
     int algo = SU2_BALGO_MIXED;
-    int key() const { return (((n_or * 3 + algo) * 2 + (int)pre) * 2 + (int)post) * 2 + (int)meas; }
+    uint32_t g_sw = 0, g_nm = 0, g_en = 0, g_fin = 0; /* workgroups per launch kind */
+    uint64_t key() const
+    {
+        auto lg = [](uint32_t v) {
+            uint64_t l = 0;
+            while ((1u << l) < v) ++l;
+            return l;
+        };
+        const uint64_t flags = (((uint64_t)(n_or * 3 + algo) * 2 + pre) * 2 + post) * 2 + meas;
+        return (flags << 24) | (lg(g_sw) << 18) | (lg(g_nm) << 12) | (lg(g_en) << 6) | lg(g_fin);
+    }
 };
+
+/* The grid of a persistent launch over nblk virtual blocks: the power of two
+ * (at least 8, one workgroup per XCD) that covers them, capped at `cap`.  A
+ * light load then dispatches tens of workgroups instead of thousands that
+ * would exit at once; the grid class is part of the plan, so each class has
+ * its own captured graph. */
+uint32_t fit_grid(uint32_t nblk, uint32_t cap)
+{
+    uint32_t g = 8;
+    while (g < nblk && g < cap) g <<= 1;
+    return std::min(g, cap);
+}
 
 } // namespace
 
@@ -159,7 +181,7 @@ struct su2_engine {
     uint32_t *hB = nullptr;
     hipEvent_t upload_done = nullptr;
     bool upload_pending = false, dirty = true;
-    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs; /* by tick_plan::key */
+    std::map<uint64_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs; /* by tick_plan::key */
     int block = 256, grid = 1024, aux_grid = 1024, en_grid = 2048, ncu = 256;
     int n_active = 0;
 
@@ -270,6 +292,10 @@ struct su2_engine {
             (s.job.algo == SU2_ALGO_HEATBATH ? hb : metro) = true;
         }
         p.algo = hb && metro ? SU2_BALGO_MIXED : (hb ? SU2_BALGO_HB : SU2_BALGO_METRO);
+        p.g_sw = fit_grid(hH->sw_blocks, (uint32_t)grid);
+        p.g_nm = fit_grid(hH->nm_blocks, (uint32_t)aux_grid);
+        p.g_en = fit_grid(hH->en_blocks, (uint32_t)en_grid);
+        p.g_fin = fit_grid(hH->n_active, (uint32_t)cfg.max_jobs);
         return p;
     }
 
@@ -278,16 +304,16 @@ struct su2_engine {
         const int f32 = cfg.precision == SU2_PREC_F32;
         const uint32_t mj = (uint32_t)cfg.max_jobs;
         hipError_t e = hipSuccess;
-        if (p.pre) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 0, aux_grid, st);
+        if (p.pre) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 0, p.g_nm, st);
         for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
             for (int par = 0; par < 2 && e == hipSuccess; ++par)
-                e = su2_launch_bsweep(f32, mu, 0, p.algo, dH, dD, dB, mj, par, 0, grid, block, st);
+                e = su2_launch_bsweep(f32, mu, 0, p.algo, dH, dD, dB, mj, par, 0, p.g_sw, block, st);
         for (int pass = 0; pass < p.n_or && e == hipSuccess; ++pass)
             for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
                 for (int par = 0; par < 2 && e == hipSuccess; ++par)
-                    e = su2_launch_bsweep(f32, mu, 1, p.algo, dH, dD, dB, mj, par, pass, grid, block, st);
-        if (p.post && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 1, aux_grid, st);
-        if (p.meas && e == hipSuccess) e = su2_launch_ben_av(f32, dH, dD, dB, mj, dctr, en_grid, st);
+                    e = su2_launch_bsweep(f32, mu, 1, p.algo, dH, dD, dB, mj, par, pass, p.g_sw, block, st);
+        if (p.post && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 1, p.g_nm, st);
+        if (p.meas && e == hipSuccess) e = su2_launch_ben_av(f32, dH, dD, dB, mj, dctr, p.g_en, p.g_fin, st);
         if (!p.meas && e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
         if (e != hipSuccess) return su2_set_error(SU2_EHIP, "tick launch: %s", hipGetErrorString(e));
         return SU2_OK;
@@ -340,6 +366,11 @@ struct su2_engine {
             if (it != graphs.end()) {
                 x = it->second.second;
             } else {
+                if (graphs.size() >= 64) {
+                    /* bound the cache: retire every graph once the queued ones ran */
+                    ENG_HIP(hipStreamSynchronize(st));
+                    drop_graph();
+                }
                 int rc = capture(p, &x);
                 if (rc) return rc;
             }

@@ -224,11 +224,19 @@ class Server:
         return jid.value
 
     def poll(self, max_results: int = 1024) -> list[Result]:
+        """Finished results since the last poll.  If the server has stopped
+        on an error, raises Su2Error carrying the results it still handed
+        over in its `results` attribute."""
         out = (Su2JobResult * max_results)()
         n = C.c_int()
-        L.check(self.lib.su2_server_poll(self.h, out, max_results, C.byref(n)), "server_poll")
-        return [Result(r.id, r.status, r.t_submit, r.t_admit, r.t_done, r.E_mean, r.E_err, r.P_mean, r.P_err,
-                       r.accept, r.link_updates) for r in out[: n.value]]
+        rc = self.lib.su2_server_poll(self.h, out, max_results, C.byref(n))
+        res = [Result(r.id, r.status, r.t_submit, r.t_admit, r.t_done, r.E_mean, r.E_err, r.P_mean, r.P_err,
+                      r.accept, r.link_updates) for r in out[: n.value]]
+        if rc != L.SU2_OK:
+            err = L.Su2Error(rc, "server_poll", self.lib.su2_last_error().decode(errors="replace"))
+            err.results = res
+            raise err
+        return res
 
     def series(self, jid: int) -> tuple[np.ndarray, np.ndarray]:
         job = self.jobs[jid]
