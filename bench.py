@@ -293,6 +293,7 @@ def batched_bench(args, precision):
     ups = sum(j.link_updates for j in jobs)
     return {"jobs": len(jobs), "sites": int(sum(j.V for j in jobs)), "sweeps_per_job": args.batch_therm + args.batch_meas,
             "link_updates": ups, "batched_s": t_batch, "one_by_one_s": t_single,
+            "batched_jobs_per_s": len(jobs) / t_batch, "one_by_one_jobs_per_s": len(jobs) / t_single,
             "batched_link_updates_per_s": ups / t_batch, "one_by_one_link_updates_per_s": ups / t_single,
             "speedup": t_single / t_batch, "autotuned_block": block, "autotuned_grid": grid,
             "results_identical": same, "precision": "FP64" if precision == 0 else "FP32 links"}
@@ -469,7 +470,8 @@ def main():
                      "frac": hb_gbs / HBM_PEAK_GBS, "traffic": load_traffic(),
                      "kernel": "heat-bath sweep launch (one direction x parity)",
                      "bytes_per_update": BYTES_HB, "updates_per_launch": per_launch_updates,
-                     "avg_launch_us": r["hb_ms"] * 1e3},
+                     "avg_launch_us": r["hb_ms"] * 1e3,
+                     "traffic_gbs": (load_traffic() / (r["hb_ms"] * 1e-3) / 1e9) if load_traffic() else None},
         "overrelax_launch": {"avg_launch_us": r["or_ms"] * 1e3, "achieved_gbs": or_gbs,
                              "updates_per_s_kernel": per_launch_updates / (r["or_ms"] * 1e-3)},
         "fp64_rate": fp64_rate(r, per_launch_updates),
