@@ -101,12 +101,16 @@ __device__ static inline bool normalize_post(const su2_bdesc &d, uint32_t s)
     return (i % (uint32_t)d.nevery) == 0u;
 }
 
+/* One link update of lattice d at half-site h.  The heat-bath runs with the
+ * workgroup-shared rejection loop, so every thread of the workgroup calls
+ * this for each virtual block (in_range = 0 past the end of the lattice);
+ * the algorithm is uniform over a virtual block. */
 template <int MU, int BC, int ALGO, typename LinkT>
-__device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t h)
+__device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t h, int in_range, su2_kp_queue &q)
 {
     if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
-        su2_sweep_site<MU, BC, SU2_K_HB, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
-    else
+        su2_sweep_site_hb_wg<MU, BC, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, in_range, q);
+    else if (in_range)
         su2_sweep_site<MU, BC, SU2_K_METRO, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
 }
 
@@ -124,12 +128,12 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
     xcd_walk(H->sw_blocks, &vb0, &vend, &step);
     if (vb0 >= vend) return;
     const bool lds = stage_begins(B, n, s_begin);
-    for (uint32_t vb = vb0; vb < vend; vb += step) {
-        const su2_bdesc &d = D[find_desc(B, s_begin, lds, n, vb)];
-        if (OR && or_pass >= d.or_sweeps) continue;
-        const uint32_t h = (vb - d.sw_begin) * blockDim.x + threadIdx.x;
-        if (h >= d.g.Vh) continue;
-        if (OR) {
+    if constexpr (OR) {
+        for (uint32_t vb = vb0; vb < vend; vb += step) {
+            const su2_bdesc &d = D[find_desc(B, s_begin, lds, n, vb)];
+            if (or_pass >= d.or_sweeps) continue;
+            const uint32_t h = (vb - d.sw_begin) * blockDim.x + threadIdx.x;
+            if (h >= d.g.Vh) continue;
             switch (d.bc) {
             case SU2_K_BC_PER:
                 su2_sweep_site<MU, SU2_K_BC_PER, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
@@ -141,11 +145,18 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
                 su2_sweep_site<MU, SU2_K_BC_PMC, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
                 break;
             }
-        } else {
+        }
+    } else {
+        __shared__ su2_kp_queue q;
+        for (uint32_t vb = vb0; vb < vend; vb += step) {
+            /* everything up to the update is uniform in the workgroup */
+            const su2_bdesc &d = D[find_desc(B, s_begin, lds, n, vb)];
+            const uint32_t h = (vb - d.sw_begin) * blockDim.x + threadIdx.x;
+            const int in_range = h < d.g.Vh;
             switch (d.bc) {
-            case SU2_K_BC_PER: site_link<MU, SU2_K_BC_PER, ALGO, LinkT>(d, parity, h); break;
-            case SU2_K_BC_PCC: site_link<MU, SU2_K_BC_PCC, ALGO, LinkT>(d, parity, h); break;
-            default: site_link<MU, SU2_K_BC_PMC, ALGO, LinkT>(d, parity, h); break;
+            case SU2_K_BC_PER: site_link<MU, SU2_K_BC_PER, ALGO, LinkT>(d, parity, h, in_range, q); break;
+            case SU2_K_BC_PCC: site_link<MU, SU2_K_BC_PCC, ALGO, LinkT>(d, parity, h, in_range, q); break;
+            default: site_link<MU, SU2_K_BC_PMC, ALGO, LinkT>(d, parity, h, in_range, q); break;
             }
         }
     }

@@ -44,13 +44,44 @@ __device__ static inline __attribute__((always_inline)) uint32_t sweep_block(uin
 }
 
 template <int MU, int BC, int UPD, typename LinkT>
+__device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *__restrict__ U,
+                                                                        uint32_t *__restrict__ rng,
+                                                                        int *__restrict__ accept, const su2_kgeom &g,
+                                                                        const su2_kphys &ph, int parity,
+                                                                        uint32_t strip, uint32_t bpp)
+{
+    const uint32_t h = sweep_block(strip, bpp) * blockDim.x + threadIdx.x;
+    if constexpr (UPD == SU2_K_HB) {
+        __shared__ su2_kp_queue q;
+        su2_sweep_site_hb_wg<MU, BC, LinkT>(U, rng, accept, g, ph, parity, h, h < g.Vh, q);
+    } else {
+        if (h >= g.Vh) return;
+        su2_sweep_site<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, h);
+    }
+}
+
+template <int MU, int BC, int UPD, typename LinkT>
 __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *__restrict__ rng,
                                                int *__restrict__ accept, su2_kgeom g, su2_kphys ph, int parity,
                                                uint32_t strip, uint32_t bpp)
 {
-    const uint32_t h = sweep_block(strip, bpp) * blockDim.x + threadIdx.x;
-    if (h >= g.Vh) return;
-    su2_sweep_site<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, h);
+    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, strip, bpp);
+}
+
+/* FP64-link sweeps run at most 4 waves per SIMD.  The heat-bath with the
+ * shared rejection loop needs only 74-108 VGPRs and over-relaxation 66, so
+ * the compiler would allow 5-7 waves, but more workgroups in flight spread
+ * each XCD's working set over more t-planes.  Measured on one box, 4 waves
+ * against uncapped: heat-bath 128^3 -4 %, 256^3 -9 % (one plane of the five
+ * link arrays is already larger than an XCD's 4 MB L2 there);
+ * over-relaxation -6 % and -5 %.  FP32 links (half the footprint) are as
+ * fast or faster uncapped. */
+template <int MU, int BC, int UPD, typename LinkT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void
+k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
+           int parity, uint32_t strip, uint32_t bpp)
+{
+    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, strip, bpp);
 }
 
 template <int MU, int BC, int UPD, typename LinkT>
@@ -64,8 +95,12 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
     if (sizeof(LinkT) == sizeof(su2_l64) && g.plane % (uint32_t)block == 0 && bpp % w == 0 && bpp <= 64 &&
         (g.Vh / (uint32_t)block) % (8u * bpp) == 0 && nb * (uint32_t)block == g.Vh)
         strip = w;
-    hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g, p,
-                       parity, strip, bpp);
+    if constexpr (sizeof(LinkT) == sizeof(su2_l64))
+        hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
+                           p, parity, strip, bpp);
+    else
+        hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
+                           p, parity, strip, bpp);
     return hipGetLastError();
 }
 

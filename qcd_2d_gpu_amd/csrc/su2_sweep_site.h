@@ -67,12 +67,13 @@ __device__ static inline __attribute__((always_inline)) uint32_t su2_hidx(const 
     return (y >> 1) + x * g.Lyh + t * g.plane;
 }
 
-template <int MU, int BC, int UPD, typename LinkT>
-__device__ static inline __attribute__((always_inline)) void su2_sweep_site(LinkT *__restrict__ U,
-                                                                            uint32_t *__restrict__ rng,
-                                                                            int *__restrict__ accept,
-                                                                            const su2_kgeom &g, const su2_kphys &ph,
-                                                                            int parity, uint32_t h)
+/* The staple A of U_mu at half-site h of parity p; returns 0 (and leaves A
+ * alone) for a link the boundary condition does not update (PCC: x and t
+ * links at y = 0). */
+template <int MU, int BC, typename LinkT>
+__device__ static inline __attribute__((always_inline)) int su2_site_staple(const LinkT *__restrict__ U,
+                                                                            const su2_kgeom &g, int parity,
+                                                                            uint32_t h, su2_q *Aout)
 {
     typedef su2_link_io<LinkT> io;
 
@@ -82,7 +83,7 @@ __device__ static inline __attribute__((always_inline)) void su2_sweep_site(Link
     const uint32_t x = q - t * g.Lx;
     const uint32_t y = 2u * yh + (((t + x) & 1u) ^ (uint32_t)parity);
 
-    if (BC == SU2_K_BC_PCC && MU != SU2_DIR_Y && y == 0u) return;
+    if (BC == SU2_K_BC_PCC && MU != SU2_DIR_Y && y == 0u) return 0;
 
     /* wrapped neighbour coordinates */
     const uint32_t yp = (y + 1u == g.Ly) ? 0u : y + 1u;
@@ -129,7 +130,22 @@ __device__ static inline __attribute__((always_inline)) void su2_sweep_site(Link
         A = su2_q_add(A, up);
         A = su2_q_add(A, dn);
     }
+    *Aout = A;
+    return 1;
+}
 
+template <int MU, int BC, int UPD, typename LinkT>
+__device__ static inline __attribute__((always_inline)) void su2_sweep_site(LinkT *__restrict__ U,
+                                                                            uint32_t *__restrict__ rng,
+                                                                            int *__restrict__ accept,
+                                                                            const su2_kgeom &g, const su2_kphys &ph,
+                                                                            int parity, uint32_t h)
+{
+    typedef su2_link_io<LinkT> io;
+
+    su2_q A;
+    if (!su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A)) return;
+    const size_t Vh = g.Vh;
     LinkT *dst = U + (size_t)(parity * 3 + MU) * g.Vh + h;
     int acc;
     if (UPD == SU2_K_OR) {
@@ -152,6 +168,86 @@ __device__ static inline __attribute__((always_inline)) void su2_sweep_site(Link
         for (int i = 0; i < 6; ++i) rng[(size_t)i * Vh + h] = s[i];
     }
     accept[(size_t)parity * Vh + h] += acc;
+}
+
+/*
+ * Heat-bath with the rejection loop shared by the workgroup.  A wavefront
+ * runs the Kennedy-Pendleton loop until its slowest lane accepts: with
+ * about 2 % of the sites rejecting their first trial, nearly every
+ * wavefront pays for a second trial (1.8 trials per wavefront against 1.02
+ * per site, 128^3 at beta = 5).  Here every site runs its first trial in
+ * place; the sites that reject it move their RNG state and scale into an
+ * LDS queue, the first lanes of the workgroup run those sites' remaining
+ * trials to acceptance, and each site takes its state and lambda back to
+ * finish its own link.  A site's stream is consumed in the same order as
+ * in su2_heat_bath, so the results are identical bit for bit.
+ *
+ * Every thread of the workgroup must call this the same number of times
+ * (it holds three barriers): threads without a site pass in_range = 0.
+ */
+struct su2_kp_queue {
+    uint32_t n;
+    uint32_t s[6][256];
+    float inv[256];
+    float lf[256];
+};
+
+template <int MU, int BC, typename LinkT>
+__device__ static inline __attribute__((always_inline)) void
+su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept,
+                     const su2_kgeom &g, const su2_kphys &ph, int parity, uint32_t h, int in_range,
+                     su2_kp_queue &q)
+{
+    typedef su2_link_io<LinkT> io;
+
+    su2_q A = su2_q_zero();
+    const int active = in_range ? su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A) : 0;
+    const size_t Vh = g.Vh;
+    uint32_t s[6];
+    float inv = 0.0f, lf = 0.0f;
+    int done = 1;
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * Vh + h];
+        inv = su2_kp_setup(&A, ph.beta);
+        done = su2_kp_trial(s, inv, &lf);
+    }
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
+    uint32_t slot = 0;
+    if (!done) {
+        slot = atomicAdd(&q.n, 1u);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) q.s[i][slot] = s[i];
+        q.inv[slot] = inv;
+    }
+    __syncthreads();
+    const uint32_t n = q.n;
+    if (threadIdx.x < n) {
+        uint32_t w[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) w[i] = q.s[i][threadIdx.x];
+        const float iw = q.inv[threadIdx.x];
+        float l;
+        while (!su2_kp_trial(w, iw, &l)) {
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) q.s[i][threadIdx.x] = w[i];
+        q.lf[threadIdx.x] = l;
+    }
+    __syncthreads();
+    if (!done) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = q.s[i][slot];
+        lf = q.lf[slot];
+    }
+    if (active) {
+        LinkT *dst = U + (size_t)(parity * 3 + MU) * g.Vh + h;
+        io::store(dst, su2_kp_finish(A, lf, s));
+#pragma unroll
+        for (int i = 0; i < 6; ++i) rng[(size_t)i * Vh + h] = s[i];
+        accept[(size_t)parity * Vh + h] += 1;
+    }
 }
 
 #endif /* SU2_SWEEP_SITE_H */

@@ -28,21 +28,35 @@
  * sqrt(1 - X0^2) exp(beta a X0) by the rejection loop, evaluated in float as
  * the reference does; the direction (cos theta, phi) is drawn in float and
  * renormalised in double; the new link is X A^^dagger.
+ * Split in three so a sweep kernel can run the trials of sites rejected at
+ * the first trial elsewhere (su2_sweep_site.h): su2_kp_setup normalises A
+ * and returns the float scale 0.5 / (a beta), su2_kp_trial is one pass of
+ * the loop, su2_kp_finish draws the direction.
  */
-SU2_U_FN su2_q su2_heat_bath(su2_q A, double beta, uint32_t *s)
+SU2_U_FN float su2_kp_setup(su2_q *A, double beta)
 {
     double a;
-    A = su2_q_normalize_det(A, &a);
-    float inv = (float)(0.5 / (a * beta));
-    float lf, aux;
-    do {
-        float r0 = su2_mrg_uniform(s);
-        float r1 = su2_mrg_uniform(s);
-        float r2 = su2_mrg_uniform(s);
-        float c = su2_cospif(2.0f * r1);
-        lf = -(su2_logf(r0) + c * c * su2_logf(r2)) * inv;
-        aux = 1.0f - su2_mrg_uniform(s);
-    } while (aux * aux > 1.0f - lf);
+    *A = su2_q_normalize_det(*A, &a);
+    return (float)(0.5 / (a * beta));
+}
+
+/* One trial of the rejection loop: four draws; returns whether it is
+ * accepted, with lambda in *lf. */
+SU2_U_FN int su2_kp_trial(uint32_t *s, float inv, float *lf)
+{
+    float r0 = su2_mrg_uniform(s);
+    float r1 = su2_mrg_uniform(s);
+    float r2 = su2_mrg_uniform(s);
+    float c = su2_cospif(2.0f * r1);
+    float l = -(su2_logf(r0) + c * c * su2_logf(r2)) * inv;
+    float aux = 1.0f - su2_mrg_uniform(s);
+    *lf = l;
+    return !(aux * aux > 1.0f - l);
+}
+
+/* The new link from an accepted lambda: two more draws for the direction. */
+SU2_U_FN su2_q su2_kp_finish(su2_q A, float lf, uint32_t *s)
+{
     double x0 = 1.0 - 2.0 * (double)lf;
     double n = su2_sqrt_rn(1.0 - x0 * x0);
     float ct = 2.0f * su2_mrg_uniform(s) - 1.0f;
@@ -53,6 +67,15 @@ SU2_U_FN su2_q su2_heat_bath(su2_q A, double beta, uint32_t *s)
     double sc = n * su2_rsqrt(v1 * v1 + v2 * v2 + v3 * v3);
     su2_q X = su2_q_make(x0, v1 * sc, v2 * sc, v3 * sc);
     return su2_q_mul_nd(X, A);
+}
+
+SU2_U_FN su2_q su2_heat_bath(su2_q A, double beta, uint32_t *s)
+{
+    float inv = su2_kp_setup(&A, beta);
+    float lf;
+    while (!su2_kp_trial(s, inv, &lf)) {
+    }
+    return su2_kp_finish(A, lf, s);
 }
 
 /*
