@@ -1,0 +1,60 @@
+"""bench.py's host side on the CPU: the one-GPU refusal, the numbers it reads
+from the committed rocprofv3 summaries, the CPU-baseline record and the
+batched-ensemble workload it draws."""
+import argparse
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+import qcd_2d_gpu_amd as q  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "4"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_profile_values_are_committed():
+    traffic = bench.load_traffic()
+    assert traffic is not None and 2.4e8 < traffic < 4.0e8  # HBM bytes per 128^3 heat-bath launch
+    assert traffic / (240.0 * 1048576) >= 1.0  # never below the algorithmic bytes
+    hb = bench.load_profile_value("sweep_128cube_flops.txt", "hb_fp64_flops_per_update")
+    orf = bench.load_profile_value("sweep_128cube_flops.txt", "or_fp64_flops_per_update")
+    assert hb is not None and orf is not None and 300 < orf < hb < 1000
+    assert bench.load_profile_value("sweep_128cube_flops.txt", "no_such_key") is None
+    r = {"hb_ms": 0.05, "or_ms": 0.04}
+    f = bench.fp64_rate(r, 1048576)
+    assert abs(f["heatbath_tflops"] - hb * 1048576 / 0.05e-3 / 1e12) < 1e-9
+    assert f["peak_tflops"] == 78.6 and 0 < f["heatbath_frac"] < 1
+
+
+def test_cpu_baseline_record():
+    args = argparse.Namespace(cpu_threads=2, L=8, cpu_steps=2)
+    out = bench.cpu_baseline(args, 8 ** 3)
+    assert set(out) >= {"value", "unit", "cores", "kind", "sample"}
+    assert out["unit"] == "link updates/s" and out["kind"] == "port" and out["cores"] == 2
+    assert out["value"] > 0 and "2 steps" in out["sample"]
+    args.cpu_steps = 0
+    assert bench.cpu_baseline(args, 8 ** 3) is None
+
+
+def test_ensemble_workload():
+    jobs = bench.ensemble_jobs(q, 64, 20, 80)
+    assert len(jobs) == 64
+    assert jobs == bench.ensemble_jobs(q, 64, 20, 80)  # same draw every run
+    for i, j in enumerate(jobs):
+        assert j.Lt == j.Lx and 8 <= j.Lt <= 48 and j.Lt % 2 == 0
+        assert 8 <= j.Ly <= 64 and j.Ly % 2 == 0
+        assert 2.0 <= j.beta <= 8.0 and j.beta == float(np.float32(j.beta))
+        assert j.bc in (q.BC_PERIODIC, q.BC_PCC, q.BC_PMC) and j.algo == q.ALGO_HEATBATH
+        assert j.seed == 1000 + i and j.n_therm == 20 and j.n_meas == 80 and j.n_cor == 1
+    assert {j.bc for j in jobs} == {0, 1, 2}
+    assert sum(j.V for j in jobs) == 1891024
