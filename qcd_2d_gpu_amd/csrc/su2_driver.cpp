@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "su2_internal.h"
 #include "su2mc.h"
 
 namespace {
@@ -32,6 +33,9 @@ const char *bound_name(int bc)
 {
     return bc == SU2_BC_PERIODIC ? "per" : (bc == SU2_BC_PCC ? "pcc" : "pmc");
 }
+
+/* SU2_EIO with the file and what failed (the reference exit(1)s) */
+int io_fail(const char *what, const std::string &f) { return su2_set_error(SU2_EIO, "%s %s", what, f.c_str()); }
 
 std::string path_of(const su2_run_config *c, const char *name)
 {
@@ -79,7 +83,7 @@ int write_autocorrelation(const su2_run_config *c, const char *prefix, const dou
     /* save_autocorrelation / save_autocorrelation_av (output.h:1-14,45-58) */
     std::string f = path_of(c, (std::string(prefix) + make_tail(l, bound, c->nsave).tail + ".txt").c_str());
     FILE *fp = fopen(f.c_str(), "w");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     fprintf(fp, "N_termalizacion %d N %d N_correlation %d  D_t %d D_x %d D_ y %d beta %.2lf deltaphi %.2lf\n", N_term,
             N, N_cor, l.t, l.x, l.y, (double)l.beta, (double)l.epsilon);
     fprintf(fp, "N Autocorrelation\n");
@@ -94,7 +98,7 @@ int write_energy(const su2_run_config *c, double mean, double var, int N_term, i
     /* save_energy (output.h:15-28) */
     std::string f = path_of(c, (std::string("Energy_") + make_tail(l, bound, c->nsave).tail + ".txt").c_str());
     FILE *fp = fopen(f.c_str(), "w");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     fprintf(fp, "N_termalizacion %d N %d N_correlation %d  D_t %d D_x %d D_ y %d deltaphi %.2lf accept %.2lf\n", N_term,
             N, N_cor, l.t, l.x, l.y, (double)l.epsilon, acc);
     fprintf(fp, "M Energy Error\n");
@@ -109,7 +113,7 @@ int write_energy_av(const su2_run_config *c, double m1, double v1, double m2, do
     /* save_energy_av (output.h:30-44) */
     std::string f = path_of(c, (std::string("Energy_av_") + make_tail(l, bound, c->nsave).tail + ".txt").c_str());
     FILE *fp = fopen(f.c_str(), "w");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     fprintf(fp,
             "N_termalizacion %d N %d N_correlation %d N_updates %d  D_t %d D_x %d D_ y %d deltaphi %.2lf accept "
             "%.2lf\n",
@@ -127,7 +131,7 @@ int write_en_seq(const su2_run_config *c, const double *plaq, const double *en, 
     /* save_en_seq (output.h:61-75): "i E P" per measurement */
     std::string f = path_of(c, (std::string("Check_") + make_tail(l, bound, c->nsave).tail + ".txt").c_str());
     FILE *fp = fopen(f.c_str(), "w");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     fprintf(fp,
             "N_termalizacion %d N %d N_correlation %d N_updates %d  D_t %d D_x %d D_y %d deltaphi %.2lf accept "
             "%.2lf\n",
@@ -145,13 +149,13 @@ int write_conf(const su2_run_config *c, const std::vector<double> &phi, const su
     names n = make_tail(l, bound, c->nconfs);
     std::string f = path_of(c, (std::string("Config_") + n.tail + ".txt").c_str());
     FILE *fp = fopen(f.c_str(), "wb");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     size_t w = fwrite(phi.data(), sizeof(double), phi.size(), fp);
     fclose(fp);
-    if (w != phi.size()) return SU2_EIO;
+    if (w != phi.size()) return io_fail("short write to", f);
     f = path_of(c, (std::string("Config_") + n.tail + "_data.txt").c_str());
     fp = fopen(f.c_str(), "w");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     fprintf(fp, "N_termalization %d Type_termalization %d", N_term, init);
     fclose(fp);
     return SU2_OK;
@@ -164,16 +168,16 @@ int read_conf(const su2_run_config *c, std::vector<double> &phi, const su2_latti
     names n = make_tail(l, bound, c->nconfr);
     std::string f = path_of(c, (std::string("Config_") + n.tail + ".txt").c_str());
     FILE *fp = fopen(f.c_str(), "rb");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     size_t r = fread(phi.data(), sizeof(double), phi.size(), fp);
     fclose(fp);
-    if (r != phi.size()) return SU2_EIO;
+    if (r != phi.size()) return io_fail("short read from", f);
     f = path_of(c, (std::string("Config_") + n.tail + "_data.txt").c_str());
     fp = fopen(f.c_str(), "r");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     int got = fscanf(fp, "N_termalization %d Type_termalization %d", Nterm, init);
     fclose(fp);
-    return got == 2 ? SU2_OK : SU2_EIO;
+    return got == 2 ? SU2_OK : io_fail("malformed", f);
 }
 
 /* RNG checkpoint (extension: the reference reseeds on resume, so its
@@ -184,10 +188,10 @@ int write_rng(const su2_run_config *c, const std::vector<uint32_t> &st, const su
     names n = make_tail(l, bound, c->nconfs);
     std::string f = path_of(c, (std::string("Config_") + n.tail + "_rng.bin").c_str());
     FILE *fp = fopen(f.c_str(), "wb");
-    if (!fp) return SU2_EIO;
+    if (!fp) return io_fail("cannot open", f);
     size_t w = fwrite(st.data(), sizeof(uint32_t), st.size(), fp);
     fclose(fp);
-    return w == st.size() ? SU2_OK : SU2_EIO;
+    return w == st.size() ? SU2_OK : io_fail("short write to", f);
 }
 
 /* 1: restored, 0: no checkpoint file, < 0: error */
@@ -199,7 +203,7 @@ int read_rng(const su2_run_config *c, std::vector<uint32_t> &st, const su2_latti
     if (!fp) return 0;
     size_t r = fread(st.data(), sizeof(uint32_t), st.size(), fp);
     fclose(fp);
-    return r == st.size() ? 1 : SU2_EIO;
+    return r == st.size() ? 1 : io_fail("short read from", f);
 }
 
 double mean_det(const std::vector<double> &phi)

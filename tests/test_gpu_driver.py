@@ -149,3 +149,38 @@ def test_strong_coupling_on_gpu(gpu):
     u = iv(2, 0.5) / iv(1, 0.5)
     m, e = q.stats.binned_error(r.plaquette, 20)
     assert abs(m - (u + 4 * u ** 5)) < 4 * e + 2e-4, (m, e, u + 4 * u ** 5)
+
+
+def test_missing_checkpoint_is_an_error(gpu, tmp_path):
+    """readconf without a Config_ file: SU2_EIO with the file named (the
+    reference exit(1)s, functions/lattice_setup.h:87-90)."""
+    q = gpu
+    from qcd_2d_gpu_amd import _lib as L
+
+    print("[driver] missing checkpoint", flush=True)
+    with pytest.raises(L.Su2Error) as e:
+        q.run(Lt=8, Lx=8, Ly=8, beta=2.0, readconf=1, N_term=2, N=2, outdir=str(tmp_path))
+    assert e.value.code == L.SU2_EIO
+    assert "Config_8_8_8_B_2.00_per_1.txt" in str(e.value)
+
+
+def test_runs_are_deterministic(gpu):
+    """Two runs from the same seed give the same bits: no atomics or
+    order-dependent sums anywhere in the sweep and the reduction."""
+    q = gpu
+    print("[driver] determinism 24x16x32 heat-bath + over-relaxation", flush=True)
+    runs = []
+    for _ in range(2):
+        g = q.Lattice(24, 16, 32, 3.5)
+        try:
+            g.setup_kernel(31)
+            g.rand_init2()
+            for _ in range(5):
+                g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
+                g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
+            runs.append((g.field_ref(), g.rng_state(), g.accept(), g.en_av()))
+        finally:
+            g.close()
+    (f1, s1, a1, o1), (f2, s2, a2, o2) = runs
+    assert np.array_equal(f1.view(np.uint64), f2.view(np.uint64))
+    assert np.array_equal(s1, s2) and np.array_equal(a1, a2) and o1 == o2
