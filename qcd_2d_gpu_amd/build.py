@@ -24,8 +24,10 @@ EXE = os.path.join(PKG, "su2mc")
 ARCH = os.environ.get("SU2MC_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-SOURCES = ["su2_sweep.hip", "su2_misc.hip", "su2_batch.hip", "su2_capi.cpp", "su2_handle.cpp", "su2_driver.cpp",
-           "su2_engine.cpp"]
+SOURCES = ["su2_sweep.hip", "su2_misc.hip", "su2_batch.hip", "su2_capi.cpp", "su2_handle.cpp", "su2_engine.cpp"]
+C_SOURCES = ["su2_driver.c"]  # host driver in C, on the C ABI
+CC = os.environ.get("CC", shutil.which("gcc") or "cc")
+CFLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-Wall", "-I", INCLUDE, "-I", CSRC]
 
 COMMON = [
     "-O3",
@@ -65,12 +67,19 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
             print(" ".join(cmd), flush=True)
         _run(cmd)
 
+    def compile_c(src: str) -> None:
+        cmd = [CC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", _obj(src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(compile_one, SOURCES))
+        list(ex.map(compile_c, C_SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *[_obj(s) for s in SOURCES]])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *[_obj(s) for s in SOURCES + C_SOURCES]])
     os.replace(tmp, LIB)
-    _run([HIPCC, "-O2", "-std=c++17", "-I", INCLUDE, os.path.join(CSRC, "su2mc_main.cpp"), "-o", EXE,
+    _run([CC, "-O2", "-std=c11", "-Wall", "-I", INCLUDE, os.path.join(CSRC, "su2mc_main.c"), "-o", EXE,
           "-L", PKG, "-lsu2mc", "-Wl,-rpath,$ORIGIN"])
     return LIB
 

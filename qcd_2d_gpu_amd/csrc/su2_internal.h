@@ -135,8 +135,7 @@ hipError_t su2_launch_from_ref(int f32, const double *phi_ref, void *U, const su
 hipError_t su2_launch_fill_i32(int *p, int value, size_t n, hipStream_t st);
 hipError_t su2_launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 
-/* Records a message for su2_last_error() on the calling thread and returns
- * code (defined in su2_capi.cpp). */
-int su2_set_error(int code, const char *fmt, ...);
+/* su2_set_error (defined in su2_capi.cpp), shared with the C host driver */
+#include "su2_host.h"
 
 #endif /* SU2_INTERNAL_H */

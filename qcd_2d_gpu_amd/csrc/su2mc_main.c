@@ -1,13 +1,12 @@
 /*
- * su2mc_main.cpp - command-line front end of su2_run: the reference's
+ * su2mc_main.c - command-line front end of su2_run (C, on the C ABI): the reference's
  * compile-time #defines (kernel.cu:10-46, README.md:37-50) as flags, e.g.
  *   su2mc --Lt 16 --Lx 16 --Ly 16 --beta 3.0 --boundary 0 --algo 1 --init 1 \
  *         --N 100 --N_term 100 --seed 1234 --save 0
  */
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <vector>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
 
 #include "su2mc.h"
 
@@ -68,7 +67,7 @@ int main(int argc, char **argv)
 #undef INT_FLAG
         if (!strcmp(k, "--beta")) c.beta = (float)atof(v);
         else if (!strcmp(k, "--epsilon")) c.epsilon = (float)atof(v);
-        else if (!strcmp(k, "--seed")) c.seed = strtoull(v, nullptr, 10);
+        else if (!strcmp(k, "--seed")) c.seed = strtoull(v, NULL, 10);
         else if (!strcmp(k, "--outdir")) c.outdir = v;
         else if (!strcmp(k, "--nsave")) c.nsave = v;
         else if (!strcmp(k, "--nconfr")) c.nconfr = v;
