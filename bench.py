@@ -35,6 +35,8 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BYTES_HB = 240.0  # per heat-bath link update, FP64 links
 BYTES_OR = 224.0  # per over-relaxation link update, FP64 links
+BYTES_METRO = 272.0  # per Metropolis link update (any number of hits), FP64 links
+BYTES_EN_AV = 96.0  # per site and measurement, FP64 links
 
 
 def parse():
@@ -151,6 +153,25 @@ def gpu_bench(args):
     for i in range(k2):
         L.check(lib.su2_event_elapsed_ms(evs[2 * i], evs[2 * i + 1], C.byref(ms)), "elapsed")
         or_ms.append(float(ms.value))
+    # en_av (partial + final) and Metropolis (eps 0.5, 4 hits) launches, for
+    # the roofline of each kernel kind
+    def timed(fn, n):
+        out = []
+        for i in range(n):
+            a, b = evs[2 * i], evs[2 * i + 1]
+            L.check(lib.su2_event_record(a, stream), "record")
+            fn(i)
+            L.check(lib.su2_event_record(b, stream), "record")
+        g.synchronize()
+        for i in range(n):
+            L.check(lib.su2_event_elapsed_ms(evs[2 * i], evs[2 * i + 1], C.byref(ms)), "elapsed")
+            out.append(float(ms.value))
+        return out
+
+    nk = min(len(evs) // 2, 6 * args.roofline_sweeps)
+    en_ms = timed(lambda i: g.en_av_into(False, res.value, res.value + 8), nk)
+    g.set_params(epsilon=0.5, updates=4)
+    me_ms = timed(lambda i: g.mc_update(0, q.ALGO_METROPOLIS, q.UPDATE_LINK, (i // 2) % 3, i % 2), nk)
     for e in evs + [ev0, ev1]:
         lib.su2_event_destroy(e)
     if args.copy_bytes:
@@ -166,7 +187,9 @@ def gpu_bench(args):
     res.free()
     lib.su2_stream_destroy(stream)
     return dict(total_ms=total_ms, wall=wall, V=V, hb_ms=float(np.mean(hb_ms)), or_ms=float(np.mean(or_ms)),
-                hb_ms_min=float(np.min(hb_ms)), E=float(np.mean(meas[:, 0])), P=det_ok)
+                hb_ms_min=float(np.min(hb_ms)), E=float(np.mean(meas[:, 0])), P=det_ok,
+                en_ms=float(np.mean(en_ms[1:] if len(en_ms) > 1 else en_ms)),
+                me_ms=float(np.mean(me_ms[6:] if len(me_ms) > 6 else me_ms)))
 
 
 def cpu_baseline(args, V_gpu):
@@ -475,6 +498,13 @@ def main():
         "overrelax_launch": {"avg_launch_us": r["or_ms"] * 1e3, "achieved_gbs": or_gbs,
                              "updates_per_s_kernel": per_launch_updates / (r["or_ms"] * 1e-3)},
         "fp64_rate": fp64_rate(r, per_launch_updates),
+        "metropolis_launch": {"hits": 4, "epsilon": 0.5, "avg_launch_us": r["me_ms"] * 1e3,
+                              "achieved_gbs": BYTES_METRO * per_launch_updates / (r["me_ms"] * 1e-3) / 1e9,
+                              "frac_of_hbm_peak": BYTES_METRO * per_launch_updates / (r["me_ms"] * 1e-3) / 1e9
+                              / HBM_PEAK_GBS},
+        "en_av_launch": {"avg_us": r["en_ms"] * 1e3,
+                         "achieved_gbs": BYTES_EN_AV * V / (r["en_ms"] * 1e-3) / 1e9,
+                         "frac_of_hbm_peak": BYTES_EN_AV * V / (r["en_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "step_algorithmic_gbs": step_bytes / (ms_step * 1e-3) / 1e9,
         "heatbath_updates_per_s_kernel": per_launch_updates / (r["hb_ms"] * 1e-3),
         "observables": {"energy": r["E"], "plaquette": r["P"]},
