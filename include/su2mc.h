@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define SU2MC_ABI_VERSION 1
+#define SU2MC_ABI_VERSION 2
 
 /* ---- status codes ---- */
 #define SU2_OK 0
@@ -406,6 +406,8 @@ typedef struct su2_run_config {
     const char *nconfs; /* counter of the Config_ file written */
     int verbose;        /* print progress like the reference */
     int device;         /* HIP device ordinal */
+    int N_block;        /* progress lines during the measurement (reference 20) */
+    int Nblock_term;    /* progress lines during the thermalisation (reference 10) */
 } su2_run_config;
 
 typedef struct su2_run_result {

@@ -61,7 +61,7 @@ class Su2RunConfig(C.Structure):
         ("precision", C.c_int), ("normalize_every", C.c_int),
         ("seed", C.c_uint64),
         ("outdir", C.c_char_p), ("nsave", C.c_char_p), ("nconfr", C.c_char_p), ("nconfs", C.c_char_p),
-        ("verbose", C.c_int), ("device", C.c_int),
+        ("verbose", C.c_int), ("device", C.c_int), ("N_block", C.c_int), ("Nblock_term", C.c_int),
     ]
 
 
@@ -167,7 +167,7 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.su2_abi_version() != 1:
+    if lib.su2_abi_version() != 2:
         raise RuntimeError("libsu2mc ABI version mismatch")
     _lib = lib
     return lib

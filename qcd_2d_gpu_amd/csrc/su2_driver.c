@@ -253,6 +253,8 @@ void su2_run_config_default(su2_run_config *c)
     c->nconfs = "1";
     c->verbose = 0;
     c->device = 0;
+    c->N_block = 20;
+    c->Nblock_term = 10;
 }
 
 /* the state of one run */
@@ -393,15 +395,23 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
     if (c->verbose) printf("Av det %.16lf\n", mean_det(h_phi, nphi));
     CK(su2_accept_clear(l, (int *)r.acc, stream));
 
-    double t0 = now_s();
+    /* progress lines (kernel.cu:156,172-176,212-216): every N_term/Nblock_term
+     * and N/N_block sweeps; a zero block prints nothing (the reference's
+     * modulus by zero) */
+    const int block_t = c->Nblock_term > 0 ? c->N_term / c->Nblock_term : 0;
+    const int block_m = c->N_block > 0 ? N / c->N_block : 0;
+    double t0 = now_s(), t_n = t0;
     if (c->readconf < 2) { /* kernel.cu:161-187 */
         for (int i = 1; i < c->N_term + 1; ++i) {
             if (i % c->normalize_every == 0) CK(su2_normalize(l, r.phi, prec, stream));
             CK(one_sweep(&r));
-            if (c->verbose && c->N_term >= 10 && i % (c->N_term / 10) == 0) {
+            if (c->verbose && block_t > 0 && i % block_t == 0) {
                 CK(su2_stream_synchronize(stream));
-                printf("%lf Time %.3f\n", (double)i / c->N_term, now_s() - t0);
+                const double t = now_s();
+                printf("%lf Time %d Left(min)%lf\n", (double)i / c->N_term, (int)(t - t_n),
+                       (t - t0) * (c->N_term - i) / i / 60);
                 fflush(stdout);
+                t_n = t;
             }
         }
         CK(su2_normalize(l, r.phi, prec, stream));
@@ -416,6 +426,7 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
     }
     CK(su2_stream_synchronize(stream));
     double t1 = now_s();
+    t_n = t1;
 
     CK(su2_accept_clear(l, (int *)r.acc, stream)); /* kernel.cu:190 */
     double energy[2] = {0, 0}, energy2[2] = {0, 0}, saccept = 0.0;
@@ -426,6 +437,13 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
         if (i % c->normalize_every == 0) CK(su2_normalize(l, r.phi, prec, stream));
         CK(su2_en_av(l, r.phi, prec, c->boundary == SU2_BC_PMC, (double *)r.end + i - 1, (double *)r.end2 + i - 1,
                      r.work, stream));
+        if (c->verbose && block_m > 0 && i % block_m == 0) {
+            CK(su2_stream_synchronize(stream));
+            const double t = now_s();
+            printf("%lf Time %d Left(min)%lf\n", (double)i / N, (int)(t - t_n), (t - t1) * (N - i) / i / 60);
+            fflush(stdout);
+            t_n = t;
+        }
         if (i % block_s == 0 && i != N) {
             CK(su2_memcpy_d2h(en, r.end, (size_t)i * sizeof(double), stream));
             CK(su2_memcpy_d2h(en2, r.end2, (size_t)i * sizeof(double), stream));

@@ -17,7 +17,7 @@ static void usage(void)
             "             [--readconf 0|1|2] [--init 0|1|2] [--sconf 0|1] [--save 0|1] [--N n] [--N_term n]\n"
             "             [--N_cor n] [--Nsave n] [--epsilon f] [--upd n] [--W n] [--or n] [--precision 0|1]\n"
             "             [--normalize_every n] [--seed n] [--outdir dir] [--nsave s] [--nconfr s]\n"
-            "             [--nconfs s] [--device n] [--quiet]\n");
+            "             [--nconfs s] [--device n] [--N_block n] [--Nblock_term n] [--quiet]\n");
 }
 
 int main(int argc, char **argv)
@@ -64,6 +64,8 @@ int main(int argc, char **argv)
         INT_FLAG("precision", precision)
         INT_FLAG("normalize_every", normalize_every)
         INT_FLAG("device", device)
+        INT_FLAG("N_block", N_block)
+        INT_FLAG("Nblock_term", Nblock_term)
 #undef INT_FLAG
         if (!strcmp(k, "--beta")) c.beta = (float)atof(v);
         else if (!strcmp(k, "--epsilon")) c.epsilon = (float)atof(v);

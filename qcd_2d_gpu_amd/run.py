@@ -30,7 +30,8 @@ def run(Lt: int = 16, Lx: int = 16, Ly: int = 16, beta: float = 3.0, boundary: i
         Nsave: int = 1, epsilon: float = 0.1, upd: int = 1, W: int = 80, or_sweeps: int = 0,
         precision: int = L.PREC_F64, normalize_every: int = 100, seed: int = 1234, readconf: int = 0,
         sconf: int = 1, save: int = 1, outdir: str = ".", nsave: str = "1", nconfr: str = "1",
-        nconfs: str = "1", verbose: bool = False, device: int = 0) -> RunResult:
+        nconfs: str = "1", verbose: bool = False, device: int = 0, N_block: int = 20,
+        Nblock_term: int = 10) -> RunResult:
     """Run one simulation with the reference's parameters (kernel.cu:10-46
     as keyword arguments).  sconf/save follow the reference: 0 = write."""
     lib = L.load()
@@ -39,7 +40,8 @@ def run(Lt: int = 16, Lx: int = 16, Ly: int = 16, beta: float = 3.0, boundary: i
     for k, v in dict(Lt=Lt, Lx=Lx, Ly=Ly, beta=beta, boundary=boundary, algo=algo, init=init, N=N, N_term=N_term,
                      N_cor=N_cor, Nsave=Nsave, epsilon=epsilon, upd=upd, W=W, or_sweeps=or_sweeps,
                      precision=precision, normalize_every=normalize_every, seed=seed, readconf=readconf,
-                     sconf=sconf, save=save, verbose=int(verbose), device=device).items():
+                     sconf=sconf, save=save, verbose=int(verbose), device=device, N_block=N_block,
+                     Nblock_term=Nblock_term).items():
         setattr(cfg, k, v)
     keep = [s.encode() for s in (outdir, nsave, nconfr, nconfs)]
     cfg.outdir, cfg.nsave, cfg.nconfr, cfg.nconfs = keep

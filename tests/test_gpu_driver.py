@@ -3,6 +3,7 @@ series against the CPU checker run on the reference's loop, the result
 files in the reference's text formats (functions/output.h), checkpoint and
 bit-exact resume, acceptance normalisation, and the su2mc executable."""
 import os
+import re
 import shutil
 import subprocess
 
@@ -117,6 +118,15 @@ def test_cli(gpu, tmp_path):
     assert out.returncode == 0, out.stderr
     assert "Plaquette" in out.stdout and "accept 1.000000" in out.stdout
     assert (tmp_path / "Check_8_8_8_B_3.00_per_1.txt").exists()
+    # progress lines in the reference's format (kernel.cu:172-176,212-216): N_term / Nblock_term and
+    # N / N_block sweeps apart, 10 + 20 by default
+    progress = [ln for ln in out.stdout.splitlines() if re.fullmatch(r"[0-9.]+ Time -?\d+ Left\(min\)[-0-9.e+]+", ln)]
+    assert len(progress) == 30, out.stdout
+    out = subprocess.run([exe, "--Lt", "8", "--Lx", "8", "--Ly", "8", "--N", "20", "--N_term", "10", "--N_block", "4",
+                          "--Nblock_term", "0"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    progress = [ln for ln in out.stdout.splitlines() if " Left(min)" in ln]
+    assert [ln.split()[0] for ln in progress] == ["0.250000", "0.500000", "0.750000", "1.000000"]
     bad = subprocess.run([exe, "--Lt", "7"], capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "even" in bad.stderr
 
