@@ -6,6 +6,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -19,7 +20,7 @@
 namespace {
 
 thread_local char g_err[512];
-int g_sweep_block = 256;
+std::atomic<int> g_sweep_block{256}; /* set by su2_set_sweep_block, read by every launch */
 
 int fail(int code, const char *fmt, ...)
 {

@@ -3,6 +3,8 @@
  * reunitarisation, the fused energy/plaquette reduction and the conversion
  * to and from the reference's on-disk field layout.
  */
+#include <mutex>
+
 #include "su2_observe.h"
 
 namespace {
@@ -170,6 +172,10 @@ __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const u
         dst[i] = src[i];
 }
 
+/* The jump tables go to each device's __constant__ memory once; set-up may
+ * be called from several host threads (a server thread and its caller), so
+ * the check and the upload are serialised. */
+std::mutex g_jump_mu;
 bool g_jump_uploaded[64];
 
 hipError_t ensure_jump_tables(void)
@@ -177,6 +183,7 @@ hipError_t ensure_jump_tables(void)
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_jump_mu);
     if (dev >= 0 && dev < 64 && g_jump_uploaded[dev]) return hipSuccess;
     e = hipMemcpyToSymbol(HIP_SYMBOL(c_jump1), su2_mrg_jump1_host, sizeof(su2_mrg_jump1_host));
     if (e != hipSuccess) return e;
