@@ -128,3 +128,47 @@ def test_size_limit():
     assert lib.su2_lattice_init(C.byref(l), 1128, 1128, 1128, 1.0, 0.1, 1) == L.SU2_EINVAL
     big = q.Job(1128, 1128, 1128, 3.0)
     assert lib.su2_job_bytes(C.byref(big.c()), 0) == 0
+
+
+from hypothesis import given, settings, strategies as st  # noqa: E402
+
+
+@settings(max_examples=300, deadline=None)
+@given(st.integers(-3, 1200), st.integers(-3, 1200), st.integers(-3, 1200))
+def test_lattice_init_accepts_exactly_the_valid_shapes(Lt, Lx, Ly):
+    lib = q.load_library()
+    l = L.Su2Lattice()
+    rc = lib.su2_lattice_init(C.byref(l), Lt, Lx, Ly, 2.0, 0.1, 1)
+    even = all(n >= 2 and n % 2 == 0 for n in (Lt, Lx, Ly))
+    if not even:
+        assert rc == L.SU2_EODD_EXTENT
+    elif Lt * Lx * Ly > 1431633920:
+        assert rc == L.SU2_EINVAL
+    else:
+        assert rc == 0
+        assert (l.t, l.x, l.y, l.V, l.d, l.dg, l.dt) == (Lt, Lx, Ly, Lt * Lx * Ly, 3, 4, 12)
+        assert lib.su2_field_bytes(C.byref(l), 0) == 96 * l.V
+        assert lib.su2_field_bytes(C.byref(l), 1) == 48 * l.V
+        assert lib.su2_rng_bytes(C.byref(l)) == 12 * l.V
+        assert lib.su2_accept_bytes(C.byref(l)) == 4 * l.V
+
+
+@settings(max_examples=300, deadline=None)
+@given(st.integers(0, 70), st.integers(0, 70), st.integers(0, 70), st.integers(-1, 3), st.integers(-1, 2),
+       st.integers(-1, 3), st.integers(-1, 5), st.integers(-1, 5), st.integers(-1, 3), st.integers(-1, 3))
+def test_job_bytes_rejects_invalid_jobs(Lt, Lx, Ly, bc, algo, start, n_therm, n_meas, n_cor, hits):
+    lib = q.load_library()
+    j = q.Job(Lt, Lx, Ly, 3.0, bc=bc, algo=algo, start=start, n_therm=n_therm, n_meas=n_meas, n_cor=n_cor,
+              hits=hits)
+    valid = (all(n >= 2 and n % 2 == 0 for n in (Lt, Lx, Ly)) and 0 <= bc <= 2 and algo in (0, 1)
+             and 0 <= start <= 2 and n_therm >= 1 and n_meas >= 1 and n_cor >= 1
+             and (algo != 0 or hits >= 1))
+    b64 = lib.su2_job_bytes(C.byref(j.c()), 0)
+    b32 = lib.su2_job_bytes(C.byref(j.c()), 1)
+    if not valid:
+        assert b64 == 0 and b32 == 0
+    else:
+        V = Lt * Lx * Ly
+        # field + RNG + acceptance + partials + two series, each 256-byte aligned
+        assert b64 >= 96 * V + 12 * V + 4 * V + 16 * n_meas and b64 % 256 == 0
+        assert b32 >= 48 * V + 12 * V + 4 * V and b32 < b64
