@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--serving-only", action="store_true", help="run only the serving stream and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=100, help="served requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-burst", type=int, default=200,
+                    help="requests submitted at once to measure the server's capacity (0, or --serve-requests 0: "
+                         "skip)")
     ap.add_argument("--fp32-meas", type=int, default=1000, help="measurements of the FP32-link check (0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the run, launch the plain copy kernel 5 times on buffers of this size "
@@ -322,22 +325,30 @@ def batched_bench(args, precision):
             "results_identical": same, "precision": "FP64" if precision == 0 else "FP32 links"}
 
 
-def serving_bench(args):
-    """Poisson stream of run requests (Lt = Lx in {16,24,32,48,64},
-    Ly in {16,32,64}, beta uniform in [2, 8], random boundary, heat-bath,
-    50 thermalisation + 200 measured sweeps) into the continuous-batching
-    server with a 200 GB budget."""
+def serving_jobs(q, n, rng):
+    """Run requests of the serving workload: Lt = Lx in {16,24,32,48,64},
+    Ly in {16,32,64}, beta uniform in [2, 8] (float), random boundary,
+    heat-bath, hot start, 50 thermalisation + 200 measured sweeps."""
+    return [q.Job(int(L), int(L), int(Ly), float(np.float32(b)), bc=int(bc), algo=q.ALGO_HEATBATH,
+                  start=q.START_HOT, seed=7000 + i, n_therm=50, n_meas=200)
+            for i, (L, Ly, b, bc) in enumerate(zip(rng.choice([16, 24, 32, 48, 64], n), rng.choice([16, 32, 64], n),
+                                                   rng.uniform(2.0, 8.0, n), rng.integers(0, 3, n)))]
+
+
+def serving_bench(args, burst=False):
+    """The serving workload into the continuous-batching server (200 GB
+    budget): a Poisson stream at --serve-rate, or with burst=True all
+    --serve-burst requests at once (the server's capacity)."""
     import threading
 
     import qcd_2d_gpu_amd as q
 
+    n = args.serve_burst if burst else args.serve_requests
     rng = np.random.default_rng(555)
-    n = args.serve_requests
     arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=n))
-    jobs = [q.Job(int(L), int(L), int(Ly), float(np.float32(b)), bc=int(bc), algo=q.ALGO_HEATBATH,
-                  start=q.START_HOT, seed=7000 + i, n_therm=50, n_meas=200)
-            for i, (L, Ly, b, bc) in enumerate(zip(rng.choice([16, 24, 32, 48, 64], n), rng.choice([16, 32, 64], n),
-                                                   rng.uniform(2.0, 8.0, n), rng.integers(0, 3, n)))]
+    if burst:
+        arrivals[:] = 0.0
+    jobs = serving_jobs(q, n, rng)
     srv = q.Server(precision=q.PREC_F64, max_jobs=256, budget_bytes=200 * 10 ** 9, use_graph=True)
     # warm the server's kernels and graph with one small request
     srv.submit(q.Job(8, 8, 8, 3.0, n_therm=2, n_meas=2))
@@ -362,10 +373,15 @@ def serving_bench(args):
     t_first = min(r.t_submit for r in res)
     t_last = max(r.t_done for r in res)
     ups = sum(r.link_updates for r in res)
-    return {"requests": n, "completed": len(res), "offered_rate_per_s": args.serve_rate,
-            "jobs_per_s": len(res) / (t_last - t_first), "link_updates_per_s": ups / (t_last - t_first),
-            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_s": float(lat.mean()), "window_s": t_last - t_first}
+    out = {"requests": n, "completed": len(res),
+           "jobs_per_s": len(res) / (t_last - t_first), "link_updates_per_s": ups / (t_last - t_first),
+           "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+           "mean_s": float(lat.mean()), "window_s": t_last - t_first}
+    if burst:
+        out["arrival"] = "all at once"
+    else:
+        out["offered_rate_per_s"] = args.serve_rate
+    return out
 
 
 def fp32_check(args):
@@ -448,7 +464,10 @@ def main():
         print(json.dumps(batched_bench(args, args.precision)))
         return
     if args.serving_only:
-        print(json.dumps(serving_bench(args)))
+        out = {"serving": serving_bench(args)}
+        if args.serve_burst:
+            out["serving_burst"] = serving_bench(args, burst=True)
+        print(json.dumps(out))
         return
     r = gpu_bench(args)
     V = r["V"]
@@ -466,6 +485,8 @@ def main():
         extra["batched_ensemble_fp32"] = batched_bench(args, 1)
     if args.serve_requests:
         extra["serving"] = serving_bench(args)
+    if args.serve_requests and args.serve_burst:
+        extra["serving_burst"] = serving_bench(args, burst=True)
     if args.fp32_meas:
         extra["fp32_links_vs_fp64"] = fp32_check(args)
     cpu = None
