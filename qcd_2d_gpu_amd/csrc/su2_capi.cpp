@@ -65,6 +65,25 @@ su2_kgeom geom(const su2_lattice *l) { return su2_make_kgeom(l->t, l->x, l->y); 
 
 hipStream_t S(void *s) { return (hipStream_t)s; }
 
+/* Coupling checks of a link update (over-relaxation uses neither).  The
+ * heat-bath's Kennedy-Pendleton loop accepts with probability ~ beta * |A|
+ * for small beta and never at beta <= 0 (its scale 0.5 / (|A| beta) is
+ * infinite or negative), so beta must be positive; a Metropolis proposal
+ * (sqrt(1 - eps^2), eps r) needs 0 <= eps <= 1. */
+int check_couplings(const su2_lattice *l, int algo, int update)
+{
+    if (update != SU2_UPDATE_LINK) return SU2_OK;
+    if (!std::isfinite(l->beta)) return fail(SU2_EINVAL, "beta is not finite");
+    if (algo == SU2_ALGO_HEATBATH && !(l->beta > 0.0f))
+        return fail(SU2_EINVAL, "heat-bath needs beta > 0 (got %g)", (double)l->beta);
+    if (algo == SU2_ALGO_METROPOLIS) {
+        if (!(l->epsilon >= 0.0f && l->epsilon <= 1.0f))
+            return fail(SU2_EINVAL, "Metropolis needs 0 <= epsilon <= 1 (got %g)", (double)l->epsilon);
+        if (l->updates < 0) return fail(SU2_EINVAL, "negative Metropolis hits");
+    }
+    return SU2_OK;
+}
+
 int sweep(const su2_lattice *l, void *phi, int prec, int *accept, uint32_t *states, int bc, int algo, int update,
           void *stream)
 {
@@ -76,9 +95,8 @@ int sweep(const su2_lattice *l, void *phi, int prec, int *accept, uint32_t *stat
     if (update != SU2_UPDATE_LINK && update != SU2_UPDATE_OVERRELAX)
         return fail(SU2_EINVAL, "unknown update %d", update);
     if (algo != SU2_ALGO_HEATBATH && algo != SU2_ALGO_METROPOLIS) return fail(SU2_EINVAL, "unknown algorithm %d", algo);
+    if ((rc = check_couplings(l, algo, update))) return rc;
     if (update == SU2_UPDATE_LINK && !states) return fail(SU2_EINVAL, "RNG states pointer is NULL");
-    if (update == SU2_UPDATE_LINK && algo == SU2_ALGO_METROPOLIS && l->updates < 0)
-        return fail(SU2_EINVAL, "negative Metropolis hits");
     const su2_kgeom g = geom(l);
     su2_kphys p;
     p.beta = (double)l->beta;
@@ -241,6 +259,7 @@ int su2_mc_update(const su2_lattice *l, void *phi, int precision, int *accept, u
     if (update != SU2_UPDATE_LINK && update != SU2_UPDATE_OVERRELAX)
         return fail(SU2_EINVAL, "unknown update %d", update);
     if (algo != SU2_ALGO_HEATBATH && algo != SU2_ALGO_METROPOLIS) return fail(SU2_EINVAL, "unknown algorithm %d", algo);
+    if ((rc = check_couplings(l, algo, update))) return rc;
     if (update == SU2_UPDATE_LINK && !states) return fail(SU2_EINVAL, "RNG states pointer is NULL");
     su2_kphys p;
     p.beta = (double)l->beta;

@@ -91,6 +91,11 @@ int check_job(const su2_job *j, int max_or)
     if (j->bc < SU2_BC_PERIODIC || j->bc > SU2_BC_PMC) return su2_set_error(SU2_EINVAL, "job boundary");
     if (j->algo != SU2_ALGO_HEATBATH && j->algo != SU2_ALGO_METROPOLIS) return su2_set_error(SU2_EINVAL, "job algo");
     if (j->algo == SU2_ALGO_METROPOLIS && j->hits < 1) return su2_set_error(SU2_EINVAL, "Metropolis hits < 1");
+    if (!std::isfinite(j->beta)) return su2_set_error(SU2_EINVAL, "beta is not finite");
+    /* the Kennedy-Pendleton loop never accepts at beta <= 0 */
+    if (j->algo == SU2_ALGO_HEATBATH && !(j->beta > 0.0f)) return su2_set_error(SU2_EINVAL, "heat-bath needs beta > 0");
+    if (j->algo == SU2_ALGO_METROPOLIS && !(j->epsilon >= 0.0f && j->epsilon <= 1.0f))
+        return su2_set_error(SU2_EINVAL, "Metropolis needs 0 <= epsilon <= 1");
     if (j->start < SU2_START_HOT || j->start > SU2_START_HAAR) return su2_set_error(SU2_EINVAL, "job start");
     return SU2_OK;
 }

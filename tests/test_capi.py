@@ -86,6 +86,32 @@ def test_invalid_arguments_rejected_before_launch():
     assert lib.su2_strerror(L.SU2_EODD_EXTENT) == b"lattice extents must be even"
 
 
+def test_couplings_that_would_never_finish_or_give_nan_are_rejected():
+    """beta <= 0 would keep the Kennedy-Pendleton loop rejecting for ever,
+    |eps| > 1 makes the Metropolis proposal NaN: refused on the host."""
+    lib = q.load_library()
+    fake = C.c_void_p(0x1000)
+    for beta, eps, algo, ok in [(0.0, 0.1, 1, False), (-1.0, 0.1, 1, False), (float("nan"), 0.1, 1, False),
+                                (float("inf"), 0.1, 0, False), (2.0, 1.5, 0, False), (2.0, -0.1, 0, False),
+                                (0.0, 0.5, 0, True), (-1.0, 0.5, 0, True), (2.0, 1.0, 0, True)]:
+        l = q.make_lattice(8, 8, 8, 1.0)
+        l.beta, l.epsilon = beta, eps
+        lp = C.byref(l)
+        # no RNG pointer: nothing can be launched; valid couplings fail on the pointer instead
+        assert lib.su2_mc_up_per(lp, fake, 0, fake, None, algo, 0, None) == L.SU2_EINVAL
+        msg = lib.su2_last_error()
+        assert lib.su2_mc_update(lp, fake, 0, fake, None, 0, algo, 0, 1, 0, None) == L.SU2_EINVAL
+        assert lib.su2_last_error() == msg
+        if ok:
+            assert b"RNG states pointer is NULL" in msg, (beta, eps, algo, msg)
+        else:
+            assert b"beta" in msg or b"epsilon" in msg, (beta, eps, algo, msg)
+    j = q.Job(8, 8, 8, 0.0)
+    assert lib.su2_job_bytes(C.byref(j.c()), 0) == 0
+    j = q.Job(8, 8, 8, 2.0, algo=q.ALGO_METROPOLIS, epsilon=1.5)
+    assert lib.su2_job_bytes(C.byref(j.c()), 0) == 0
+
+
 def test_no_torch_in_product_package():
     """The product talks to HIP through libsu2mc only."""
     pkg = os.path.join(ROOT, "qcd_2d_gpu_amd")
