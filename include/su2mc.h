@@ -126,7 +126,11 @@ int su2_haar_init(const su2_lattice *l, void *phi, int precision, uint32_t *stat
  * the order (y,even) (y,odd) (x,even) (x,odd) (t,even) (t,odd).
  * algo = SU2_ALGO_HEATBATH or SU2_ALGO_METROPOLIS (the reference's compile-
  * time ALGO, kernel.cu:13); update = SU2_UPDATE_LINK or SU2_UPDATE_OVERRELAX
- * (the reference's `int update`).  accept[site] += accepted hits. */
+ * (the reference's `int update`).  accept[site] += accepted hits.
+ * Couplings (from *l): the heat-bath needs a finite beta > 0 (its
+ * Kennedy-Pendleton loop accepts with probability of order beta |A| for
+ * small beta, so runs below beta ~ 0.1 get slow, and never at beta <= 0);
+ * Metropolis needs 0 <= epsilon <= 1; otherwise SU2_EINVAL. */
 int su2_mc_up_per(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int algo,
                   int update, void *stream);
 
