@@ -230,7 +230,8 @@ int su2_ctx_synchronize(su2_ctx *ctx);
 
 /* Allocates an Lt x Lx x Ly lattice (kernel.cu:58,77-82,145); SU2_EODD_EXTENT
  * for odd extents (kernel.cu:61-64).  bc = SU2_BC_*, precision = SU2_PREC_*.
- * The field and RNG are uninitialised until su2_rng_init / su2_init. */
+ * The field and RNG are uninitialised until su2_rng_init / su2_init (or
+ * the uploads); the acceptance counters start at zero. */
 int su2_lattice_create(su2_ctx *ctx, int Lt, int Lx, int Ly, int bc, int precision, su2_lat **out);
 int su2_lattice_destroy(su2_lat *h);
 /* The lattice's su2_lattice struct (beta, epsilon, updates as last swept),

@@ -31,6 +31,7 @@
  * FP32-link mode (an extension of the build): every store into the field is
  * rounded to float; all arithmetic stays in double.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -345,7 +346,9 @@ int orc_heat_bath(double *A, double *phi, double beta, orc_rng *state) /* mc_upd
         float c = su2_cospif(2 * r1);
         lf = -(su2_logf(r0) + c * c * su2_logf(r2)) * inv;
         aux = 1 - orc_uniform(state);
-    } while (aux * aux > 1 - lf);
+        /* a zero or non-finite staple (inv infinite or NaN) would be rejected
+         * for ever, as in the reference; the loop ends after one trial */
+    } while (aux * aux > 1 - lf && inv <= FLT_MAX);
     X[0] = 1 - 2 * (double)lf;
     double n = sqrt(1 - X[0] * X[0]);
     float ct = 2 * orc_uniform(state) - 1, sp, cp;

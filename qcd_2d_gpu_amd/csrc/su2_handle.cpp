@@ -135,6 +135,13 @@ int su2_lattice_create(su2_ctx *c, int Lt, int Lx, int Ly, int bc, int precision
         delete h;
         return su2_set_error(SU2_ENOMEM, "lattice %dx%dx%d buffers", Lt, Lx, Ly);
     }
+    /* acceptance counters start at zero, whatever start the caller uses */
+    int rc = su2_accept_clear(&h->l, h->accept, c->st);
+    if (rc != SU2_OK) {
+        release(h);
+        delete h;
+        return rc;
+    }
     ++c->lattices;
     *out = h;
     return SU2_OK;

@@ -12,6 +12,8 @@
 #ifndef SU2_UPDATE_H
 #define SU2_UPDATE_H
 
+#include <float.h>
+
 #include "su2_fmath.h"
 #include "su2_mrg32k3a.h"
 #include "su2_quat.h"
@@ -40,8 +42,12 @@ SU2_U_FN float su2_kp_setup(su2_q *A, double beta)
     return (float)(0.5 / (a * beta));
 }
 
-/* One trial of the rejection loop: four draws; returns whether it is
- * accepted, with lambda in *lf. */
+/* One trial of the rejection loop: four draws; returns whether it ends the
+ * loop, with lambda in *lf.  A zero or non-finite staple makes the scale inv
+ * infinite or NaN, and every trial would be rejected for ever (the
+ * reference hangs there); the loop then ends after its first trial, with a
+ * NaN link, instead of hanging the device.  For every finite scale this is
+ * the reference's test. */
 SU2_U_FN int su2_kp_trial(uint32_t *s, float inv, float *lf)
 {
     float r0 = su2_mrg_uniform(s);
@@ -51,7 +57,7 @@ SU2_U_FN int su2_kp_trial(uint32_t *s, float inv, float *lf)
     float l = -(su2_logf(r0) + c * c * su2_logf(r2)) * inv;
     float aux = 1.0f - su2_mrg_uniform(s);
     *lf = l;
-    return !(aux * aux > 1.0f - l);
+    return !(aux * aux > 1.0f - l) || !(inv <= FLT_MAX);
 }
 
 /* The new link from an accepted lambda: two more draws for the direction. */

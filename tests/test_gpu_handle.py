@@ -152,3 +152,22 @@ def test_handle_cold_start_and_errors(ctx):
         assert lib.su2_ctx_destroy(c) == L.SU2_EINVAL  # a lattice is still alive
     finally:
         h.close()
+
+
+def test_zero_field_sweep_ends(ctx):
+    """A heat-bath sweep over an all-zero field (zero staples: the
+    Kennedy-Pendleton scale is infinite or NaN) ends after one trial per
+    site with NaN links instead of looping for ever; the RNG of every site
+    advances by exactly 6 draws per launch it takes part in."""
+    lib, c = ctx
+    _say("[handle] zero field heat-bath")
+    h = Handle(lib, c, 4, 4, 4, 0, 0)
+    try:
+        _ok(lib, lib.su2_rng_init(h.h, 3), "rng_init")
+        zero = np.zeros(12 * h.V)
+        _ok(lib, lib.su2_field_upload(h.h, zero.ctypes.data), "field_upload")
+        _ok(lib, lib.su2_sweep(h.h, L.ALGO_HEATBATH, L.UPDATE_LINK, 2.0, 0.1, 1), "sweep")
+        assert np.isnan(h.field()).all()
+        assert h.accept()[0] == 3 * h.V
+    finally:
+        h.close()

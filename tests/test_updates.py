@@ -47,6 +47,25 @@ def test_heat_bath_product_equals_checker(oracle_mod):
         assert list(map(int, st)) == r.state, k
 
 
+def test_heat_bath_with_zero_or_nan_staple_ends(oracle_mod):
+    """A zero staple (a field that was never initialised) or a NaN one makes
+    the Kennedy-Pendleton scale infinite or NaN: the loop must end after one
+    trial (4 + 2 draws) in the product and the checker alike, not hang."""
+    lib, shim = oracle_mod.load(), _shim.load()
+    for A in (np.zeros(4), np.full(4, np.nan), np.array([1e-300, 0.0, 0.0, 0.0])):
+        for beta in (1.0, 3.0):
+            r, st = _state(oracle_mod, 5, 9)
+            ref = oracle_mod.Rng(5, 9)
+            for _ in range(6):
+                ref.next()
+            a1, out1 = A.copy(), np.zeros(4)
+            lib.orc_heat_bath(a1.ctypes.data, out1.ctypes.data, beta, C.byref(r.st))
+            out2 = np.zeros(4)
+            shim.shim_heat_bath(A.ctypes.data, beta, st.ctypes.data, out2.ctypes.data)
+            assert r.state == ref.state and list(map(int, st)) == ref.state
+            assert np.isnan(out1).all() and np.isnan(out2).all()
+
+
 def test_metropolis_product_equals_checker(oracle_mod):
     lib, shim = oracle_mod.load(), _shim.load()
     rng = np.random.default_rng(4)
