@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--roofline-sweeps", type=int, default=20)
     ap.add_argument("--large-L", type=int, default=256, help="extent of the DRAM-bound timing (0: skip)")
+    ap.add_argument("--shard-shares", type=int, default=1,
+                    help="with the --large-L timing, time the launches on one GPU's share of a t-slab "
+                         "decomposition over 8 GPUs (0: skip)")
     ap.add_argument("--batch-jobs", type=int, default=64, help="lattices of the batched ensemble (0: skip)")
     ap.add_argument("--batch-therm", type=int, default=20)
     ap.add_argument("--batch-meas", type=int, default=80)
@@ -227,27 +230,26 @@ def cpu_baseline(args, V_gpu):
     return out
 
 
-def large_lattice_bench(args):
-    """Heat-bath and over-relaxation launches at L^3 (256^3: 1.6 GB of FP64
-    links, far beyond the 256 MB Infinity Cache, so DRAM-bound), timed per
-    launch with HIP events on the launch stream."""
+def lattice_launch_bench(Lt, Lx, Ly, reps=3):
+    """Heat-bath and over-relaxation launches of one Lt x Lx x Ly lattice
+    (FP64 links, periodic, beta = 5), each timed with HIP events on the
+    launch stream after two warm-up sweeps."""
     import qcd_2d_gpu_amd as q
     from qcd_2d_gpu_amd import _lib as L
 
     lib = q.load_library()
     stream = C.c_void_p()
     L.check(lib.su2_stream_create(C.byref(stream)), "stream_create")
-    n = args.large_L
-    g = q.Lattice(n, n, n, 5.0, precision=q.PREC_F64, stream=stream)
+    g = q.Lattice(Lt, Lx, Ly, 5.0, precision=q.PREC_F64, stream=stream)
     g.setup_kernel(7)
     g.rand_init2()
     for _ in range(2):
         g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
         g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
-    out = {"L": n, "sites": g.V}
+    out = {"shape": f"{Lt}x{Lx}x{Ly}", "sites": g.V}
     for upd, name, nbytes in ((q.UPDATE_LINK, "heatbath", BYTES_HB), (q.UPDATE_OVERRELAX, "overrelax", BYTES_OR)):
         evs = []
-        for _ in range(3):
+        for _ in range(reps):
             for mu in range(3):
                 for par in range(2):
                     a, b = C.c_void_p(), C.c_void_p()
@@ -269,6 +271,29 @@ def large_lattice_bench(args):
                      "frac_of_hbm_peak": nbytes * g.Vh / t / 1e9 / HBM_PEAK_GBS}
     g.close()
     lib.su2_stream_destroy(stream)
+    return out
+
+
+def large_lattice_bench(args):
+    """L^3 launches (256^3: 1.6 GB of FP64 links, far beyond the 256 MB
+    Infinity Cache, so DRAM-bound)."""
+    out = lattice_launch_bench(args.large_L, args.large_L, args.large_L)
+    out["L"] = args.large_L
+    return out
+
+
+# One GPU's share of a lattice slab-decomposed along t over 8 GPUs: the
+# per-GPU kernel of the sharded scheme at its real shape (the ghost-plane
+# exchange between launches is not part of this build).
+SHARD_SHARES = ((16, 128, 128, "128^3 over 8 GPUs"), (32, 256, 256, "256^3 over 8 GPUs"))
+
+
+def shard_share_bench():
+    out = []
+    for Lt, Lx, Ly, of in SHARD_SHARES:
+        r = lattice_launch_bench(Lt, Lx, Ly)
+        r["share_of"] = of
+        out.append(r)
     return out
 
 
@@ -480,6 +505,8 @@ def main():
     extra = {}
     if args.large_L:
         extra["large_lattice_fp64"] = large_lattice_bench(args)
+    if args.large_L and args.shard_shares:
+        extra["shard_share_fp64"] = shard_share_bench()
     if args.batch_jobs:
         extra["batched_ensemble_fp64"] = batched_bench(args, 0)
         extra["batched_ensemble_fp32"] = batched_bench(args, 1)

@@ -22,21 +22,43 @@
 namespace {
 
 /* Workgroup -> block of 256 half-sites.  The hardware deals workgroups
- * round-robin to the 8 XCDs; XCD k is given the k-th contiguous eighth of
- * the lattice (a slab of t-planes).  With strip > 0 (the host sets it only
- * when every plane is a whole number of blocks, the slabs are whole planes
- * and the plane's bpp blocks split into strips of `strip` blocks), the XCD
- * walks its slab strip by strip, all its t-planes for one strip of x-rows
- * before the next: the t-neighbour rows of a strip are then re-read from
- * L2 a few hundred KB later instead of two full planes (2.6 MB at 128^3)
- * later.  Measured with FP64 links at 128^3: over-relaxation launch -10 %,
- * heat-bath -5 %; FP32 links (half the footprint) and planes of more than
- * 64 blocks (256^3, DRAM-bound) gain nothing from it and keep the plain
- * slab order. */
-__device__ static inline __attribute__((always_inline)) uint32_t sweep_block(uint32_t strip, uint32_t bpp)
+ * round-robin to the 8 XCDs, each with its own 4 MB L2; the block order
+ * decides which part of the lattice an XCD walks, and in which order.
+ *
+ * ORDER_SLAB: XCD k takes the k-th contiguous eighth of the lattice (a slab
+ * of t-planes) and walks it in memory order.
+ *
+ * ORDER_STRIPS (the host sets it only when every plane is a whole number
+ * of blocks, the slabs are whole planes and the plane's bpp blocks split
+ * into strips of `strip` blocks): the XCD walks its slab strip by strip,
+ * all its t-planes for one strip of x-rows before the next, so the
+ * t-neighbour rows of a strip are re-read from L2 a few hundred KB later
+ * instead of two full planes (2.6 MB at 128^3) later.  Measured with FP64
+ * links at 128^3: over-relaxation launch -10 %, heat-bath -5 %; FP32 links
+ * (half the footprint) and planes of more than 64 blocks (256^3,
+ * DRAM-bound) gain nothing from it and keep the slab order.
+ *
+ * ORDER_COLUMNS (thin lattices, Lt <= 32, with bpp a multiple of 8): a slab
+ * of Lt/8 <= 4 planes would send half or more of an XCD's t-neighbour reads
+ * to another XCD's planes.  Instead XCD k takes the k-th eighth of every
+ * plane's blocks (a column of x-rows through all t) and walks it t-major;
+ * only the x-neighbour rows at the column's two edges belong to other
+ * XCDs.  Measured with FP64 links against ORDER_SLAB: 32x256x256
+ * heat-bath -10 %, over-relaxation -30 %; 16x256x256 -13 % and -25 %
+ * (one GPU's share of 256^3 slab-decomposed over 8 or 16 GPUs); on 256^3
+ * itself it was 2.5 % slower. */
+enum { ORDER_SLAB = 0, ORDER_STRIPS = 1, ORDER_COLUMNS = 2 };
+
+__device__ static inline __attribute__((always_inline)) uint32_t sweep_block(uint32_t order, uint32_t strip,
+                                                                             uint32_t bpp)
 {
     const uint32_t b = blockIdx.x, xcd = b & 7u, j = b >> 3, per = gridDim.x >> 3;
-    if (strip == 0u) return xcd * per + j;
+    if (order == ORDER_COLUMNS) {
+        const uint32_t W = bpp >> 3; /* blocks of a plane in this XCD's column */
+        const uint32_t tl = j / W;
+        return tl * bpp + xcd * W + (j - tl * W);
+    }
+    if (order == ORDER_SLAB) return xcd * per + j;
     const uint32_t T = per / bpp; /* t-planes of this XCD's slab */
     const uint32_t s = j / (T * strip), r = j - s * (T * strip);
     const uint32_t tl = r / strip, c = r - tl * strip;
@@ -48,9 +70,10 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
                                                                         uint32_t *__restrict__ rng,
                                                                         int *__restrict__ accept, const su2_kgeom &g,
                                                                         const su2_kphys &ph, int parity,
-                                                                        uint32_t strip, uint32_t bpp)
+                                                                        uint32_t order, uint32_t strip,
+                                                                        uint32_t bpp)
 {
-    const uint32_t h = sweep_block(strip, bpp) * blockDim.x + threadIdx.x;
+    const uint32_t h = sweep_block(order, strip, bpp) * blockDim.x + threadIdx.x;
     if constexpr (UPD == SU2_K_HB) {
         __shared__ su2_kp_queue q;
         su2_sweep_site_hb_wg<MU, BC, LinkT>(U, rng, accept, g, ph, parity, h, h < g.Vh, q);
@@ -63,9 +86,9 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
 template <int MU, int BC, int UPD, typename LinkT>
 __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *__restrict__ rng,
                                                int *__restrict__ accept, su2_kgeom g, su2_kphys ph, int parity,
-                                               uint32_t strip, uint32_t bpp)
+                                               uint32_t order, uint32_t strip, uint32_t bpp)
 {
-    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, strip, bpp);
+    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp);
 }
 
 /* FP64-link sweeps run at most 4 waves per SIMD.  The heat-bath with the
@@ -79,9 +102,9 @@ __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *
 template <int MU, int BC, int UPD, typename LinkT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void
 k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
-           int parity, uint32_t strip, uint32_t bpp)
+           int parity, uint32_t order, uint32_t strip, uint32_t bpp)
 {
-    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, strip, bpp);
+    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp);
 }
 
 template <int MU, int BC, int UPD, typename LinkT>
@@ -89,18 +112,24 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
                       int block, hipStream_t st)
 {
     const uint32_t nb = su2_grid_blocks(g.Vh, (uint32_t)block);
-    /* strip order for FP64 links when the lattice tiles evenly */
-    uint32_t strip = 0, bpp = g.plane / (uint32_t)block;
+    /* block order for FP64 links when the lattice tiles evenly (see
+     * sweep_block); FP32 links keep the slab order */
+    uint32_t order = ORDER_SLAB, strip = 0, bpp = g.plane / (uint32_t)block;
     const uint32_t w = 8;
-    if (sizeof(LinkT) == sizeof(su2_l64) && g.plane % (uint32_t)block == 0 && bpp % w == 0 && bpp <= 64 &&
-        (g.Vh / (uint32_t)block) % (8u * bpp) == 0 && nb * (uint32_t)block == g.Vh)
+    const bool f64 = sizeof(LinkT) == sizeof(su2_l64);
+    const bool whole = g.plane % (uint32_t)block == 0 && nb * (uint32_t)block == g.Vh;
+    if (f64 && whole && g.Lt <= 32u && bpp % 8u == 0) {
+        order = ORDER_COLUMNS;
+    } else if (f64 && whole && bpp % w == 0 && bpp <= 64 && (g.Vh / (uint32_t)block) % (8u * bpp) == 0) {
+        order = ORDER_STRIPS;
         strip = w;
+    }
     if constexpr (sizeof(LinkT) == sizeof(su2_l64))
         hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
-                           p, parity, strip, bpp);
+                           p, parity, order, strip, bpp);
     else
         hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
-                           p, parity, strip, bpp);
+                           p, parity, order, strip, bpp);
     return hipGetLastError();
 }
 

@@ -120,6 +120,14 @@ def test_ragged_and_minimal_shapes(gpu, oracle_mod, shape):
              eps=0.4, hits=2)
 
 
+@pytest.mark.parametrize("shape,bc", [((8, 64, 64), 1), ((32, 64, 64), 0), ((16, 64, 128), 2)])
+def test_thin_lattices_column_block_order(gpu, oracle_mod, shape, bc):
+    """Lattices of at most 32 t-planes whose planes are whole multiples of 8
+    blocks take the x-column block order (each XCD a column of x-rows through
+    all t); FP64 links, link updates and over-relaxation."""
+    run_pair(gpu, oracle_mod, "columns", *shape, 4.0, bc, 1, 0, 31, 2, updates=(0, 1))
+
+
 @pytest.mark.parametrize("start", [1, 2])
 def test_cold_and_haar_starts(gpu, oracle_mod, start):
     for bc in (0, 1, 2):
