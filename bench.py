@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--shard-shares", type=int, default=1,
                     help="with the --large-L timing, time the launches on one GPU's share of a t-slab "
                          "decomposition over 8 GPUs (0: skip)")
+    ap.add_argument("--small-configs", type=int, default=1,
+                    help="run the 16^3 and 32x32x48 configurations through the C driver, with the checker "
+                         "timed beside them (0, or --cpu-steps 0: skip)")
     ap.add_argument("--batch-jobs", type=int, default=64, help="lattices of the batched ensemble (0: skip)")
     ap.add_argument("--batch-therm", type=int, default=20)
     ap.add_argument("--batch-meas", type=int, default=80)
@@ -297,6 +300,49 @@ def shard_share_bench():
     return out
 
 
+# The reference's small configurations, run end to end through the C host
+# driver su2_run: start, thermalisation, measurements with en_av, statistics.
+SMALL_CONFIGS = (
+    ("C1", "16^3 periodic, heat-bath, beta 3.0, cold start, seed 1234",
+     dict(Lt=16, Lx=16, Ly=16, beta=3.0, boundary=0, algo=1, init=1, seed=1234, epsilon=0.1, upd=1)),
+    ("C3_pcc", "32x32x48 PCC, Metropolis eps 0.5, 4 hits, beta 4.0, hot start, seed 4321",
+     dict(Lt=32, Lx=32, Ly=48, beta=4.0, boundary=1, algo=0, init=0, seed=4321, epsilon=0.5, upd=4)),
+    ("C3_pmc", "32x32x48 PMC, Metropolis eps 0.5, 4 hits, beta 4.0, hot start, seed 4321",
+     dict(Lt=32, Lx=32, Ly=48, beta=4.0, boundary=2, algo=0, init=0, seed=4321, epsilon=0.5, upd=4)),
+)
+
+
+def small_config_bench(args):
+    """Each small configuration: 100 thermalisation + 100 measured sweeps
+    through su2_run on the GPU (its own clock around the two loops, so
+    normalisation, en_av, progress synchronisations and copies are inside),
+    and the checker's 200 sweeps of the same lattice, start and update on
+    the host cores."""
+    import qcd_2d_gpu_amd as q
+    from oracle import oracle as O
+
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    n_t = n_m = 100
+    out = {}
+    for name, desc, kw in SMALL_CONFIGS:
+        q.run(N=n_m, N_term=n_t, **kw)  # first run: allocations and code paths warm
+        r = q.run(N=n_m, N_term=n_t, **kw)
+        gsec = r.seconds_therm + r.seconds_meas
+        V = kw["Lt"] * kw["Lx"] * kw["Ly"]
+        o = O.Oracle(kw["Lt"], kw["Lx"], kw["Ly"], kw["beta"], kw["epsilon"], kw["upd"])
+        o.setup_kernel(kw["seed"])
+        o.start(kw["init"], kw["boundary"])
+        o.time_sweeps(kw["boundary"], kw["algo"], 0, 2)
+        csec = o.time_sweeps(kw["boundary"], kw["algo"], 0, n_t + n_m)
+        gpu = r.link_updates / gsec
+        cpu = 3 * V * (n_t + n_m) / csec
+        out[name] = {"workload": desc, "sweeps": n_t + n_m, "gpu_link_updates_per_s": gpu, "gpu_seconds": gsec,
+                     "cpu_link_updates_per_s": cpu, "cpu_seconds": csec, "cpu_cores": threads,
+                     "gpu_over_cpu": gpu / cpu, "P_mean": r.P_mean, "P_err": r.P_err, "accept": r.accept}
+    return out
+
+
 def ensemble_jobs(q, n, therm, meas):
     """The batched-ensemble workload: Lt = Lx uniform over the even values
     8..48, Ly uniform over the even values 8..64, beta uniform in [2, 8]
@@ -342,12 +388,37 @@ def batched_bench(args, precision):
         same &= bool(np.array_equal(r.energy.view(np.uint64), series[s]["E"].view(np.uint64)) and
                      np.array_equal(r.plaquette.view(np.uint64), series[s]["P"].view(np.uint64)))
     ups = sum(j.link_updates for j in jobs)
-    return {"jobs": len(jobs), "sites": int(sum(j.V for j in jobs)), "sweeps_per_job": args.batch_therm + args.batch_meas,
-            "link_updates": ups, "batched_s": t_batch, "one_by_one_s": t_single,
-            "batched_jobs_per_s": len(jobs) / t_batch, "one_by_one_jobs_per_s": len(jobs) / t_single,
-            "batched_link_updates_per_s": ups / t_batch, "one_by_one_link_updates_per_s": ups / t_single,
-            "speedup": t_single / t_batch, "autotuned_block": block, "autotuned_grid": grid,
-            "results_identical": same, "precision": "FP64" if precision == 0 else "FP32 links"}
+    out = {"jobs": len(jobs), "sites": int(sum(j.V for j in jobs)), "sweeps_per_job": args.batch_therm + args.batch_meas,
+           "link_updates": ups, "batched_s": t_batch, "one_by_one_s": t_single,
+           "batched_jobs_per_s": len(jobs) / t_batch, "one_by_one_jobs_per_s": len(jobs) / t_single,
+           "batched_link_updates_per_s": ups / t_batch, "one_by_one_link_updates_per_s": ups / t_single,
+           "speedup": t_single / t_batch, "autotuned_block": block, "autotuned_grid": grid,
+           "results_identical": same, "precision": "FP64" if precision == 0 else "FP32 links"}
+    if precision == 0 and args.cpu_steps != 0:
+        out["cpu_sample"] = ensemble_cpu_sample(args, jobs[:8])
+    return out
+
+
+def ensemble_cpu_sample(args, jobs):
+    """The checker on the host cores, job after job, over the sweeps of each
+    job (start included, measurements left out): a bounded sample of the
+    ensemble's CPU rate."""
+    from oracle import oracle as O
+
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    sec, ups = 0.0, 0
+    for j in jobs:
+        o = O.Oracle(j.Lt, j.Lx, j.Ly, j.beta)
+        t0 = time.perf_counter()
+        o.setup_kernel(j.seed)
+        o.start(0, j.bc)
+        o.time_sweeps(j.bc, 1, 0, j.n_therm + j.n_meas)
+        sec += time.perf_counter() - t0
+        ups += j.link_updates
+    return {"value": ups / sec, "unit": "link updates/s", "cores": threads, "kind": "port",
+            "jobs_per_s": len(jobs) / sec,
+            "sample": f"the first {len(jobs)} jobs of the ensemble, {ups} link updates, {sec:.2f} s"}
 
 
 def serving_jobs(q, n, rng):
@@ -507,6 +578,8 @@ def main():
         extra["large_lattice_fp64"] = large_lattice_bench(args)
     if args.large_L and args.shard_shares:
         extra["shard_share_fp64"] = shard_share_bench()
+    if args.small_configs and args.cpu_steps != 0:
+        extra["small_configs"] = small_config_bench(args)
     if args.batch_jobs:
         extra["batched_ensemble_fp64"] = batched_bench(args, 0)
         extra["batched_ensemble_fp32"] = batched_bench(args, 1)

@@ -58,3 +58,26 @@ def test_ensemble_workload():
         assert j.seed == 1000 + i and j.n_therm == 20 and j.n_meas == 80 and j.n_cor == 1
     assert {j.bc for j in jobs} == {0, 1, 2}
     assert sum(j.V for j in jobs) == 1891024
+
+
+def test_ensemble_cpu_sample_record():
+    jobs = bench.ensemble_jobs(q, 64, 20, 80)
+    args = argparse.Namespace(cpu_threads=2, cpu_steps=-1)
+    out = bench.ensemble_cpu_sample(args, jobs[:1])
+    assert out["unit"] == "link updates/s" and out["kind"] == "port" and out["cores"] == 2
+    assert out["value"] > 0 and out["jobs_per_s"] > 0 and "first 1 jobs" in out["sample"]
+
+
+def test_small_configs_are_the_reference_configurations():
+    names = [c[0] for c in bench.SMALL_CONFIGS]
+    assert names == ["C1", "C3_pcc", "C3_pmc"]
+    for name, desc, kw in bench.SMALL_CONFIGS:
+        assert all(kw[k] % 2 == 0 for k in ("Lt", "Lx", "Ly"))
+        q.make_lattice(kw["Lt"], kw["Lx"], kw["Ly"], kw["beta"])
+    c1 = bench.SMALL_CONFIGS[0][2]
+    assert (c1["Lt"], c1["Lx"], c1["Ly"], c1["beta"], c1["boundary"], c1["algo"], c1["init"], c1["seed"]) == \
+        (16, 16, 16, 3.0, q.BC_PERIODIC, q.ALGO_HEATBATH, q.START_COLD, 1234)
+    for _, _, kw in bench.SMALL_CONFIGS[1:]:
+        assert (kw["Lt"], kw["Lx"], kw["Ly"], kw["beta"], kw["algo"], kw["epsilon"], kw["upd"]) == \
+            (32, 32, 48, 4.0, q.ALGO_METROPOLIS, 0.5, 4)
+    assert {kw["boundary"] for _, _, kw in bench.SMALL_CONFIGS[1:]} == {q.BC_PCC, q.BC_PMC}
