@@ -175,3 +175,37 @@ def test_single_launch_entry(gpu, oracle_mod):
                 _compare(g, o, f"launch mu={mu} parity={par}")
     finally:
         g.close()
+
+
+def _random_cases(n, seed=20261203):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(n):
+        Lt, Lx, Ly = (int(2 * rng.integers(1, 13)) for _ in range(3))
+        algo = int(rng.integers(0, 2))
+        beta = float(np.float32(rng.uniform(0.3, 8.0) if algo == 1 else rng.uniform(0.1, 8.0)))
+        cases.append(dict(i=i, shape=(Lt, Lx, Ly), bc=int(rng.integers(0, 3)), algo=algo, beta=beta,
+                          eps=float(np.float32(rng.uniform(0.0, 1.0))), hits=int(rng.integers(1, 5)),
+                          prec=int(rng.integers(0, 2)), start=int(rng.integers(0, 3)),
+                          seed=int(rng.integers(0, 2 ** 63)),
+                          updates=((0,), (0, 1))[int(rng.integers(0, 2))],
+                          block=int((64, 128, 192, 256)[int(rng.integers(0, 4))])))
+    return cases
+
+
+@pytest.mark.parametrize("case", _random_cases(24), ids=lambda c: f"case{c['i']}")
+def test_randomised_cases(gpu, oracle_mod, case):
+    """Seeded random draws over extents (even, 2..24), boundary, algorithm,
+    couplings (beta, epsilon in [0, 1], 1-4 hits), start, link precision,
+    over-relaxation and the sweep block size (64..256 threads): three sweeps
+    each, bit-exact against the checker."""
+    lib = gpu.load_library()
+    c = case
+    _say(f"[parity] random case {c}")
+    assert lib.su2_set_sweep_block(c["block"]) == 0
+    try:
+        run_pair(gpu, oracle_mod, f"random {c['i']}", *c["shape"], c["beta"], bc=c["bc"], algo=c["algo"],
+                 start=c["start"], seed=c["seed"], nsweeps=3, prec=c["prec"], eps=c["eps"], hits=c["hits"],
+                 updates=c["updates"], normalize_at=(2,))
+    finally:
+        lib.su2_set_sweep_block(0)
