@@ -99,6 +99,22 @@ def test_c2_shape_heatbath_overrelax_hot(gpu, oracle_mod):
              updates=(0, 1), compare_every=5)
 
 
+def test_headline_workload_at_full_size(gpu, oracle_mod):
+    """The measured workload itself: 128^3, periodic, beta 5, hot start, seed
+    7, heat-bath + over-relaxation steps with a reunitarisation, every link,
+    stream and counter of the 2 097 152-site lattice compared after each
+    step."""
+    run_pair(gpu, oracle_mod, "128^3", 128, 128, 128, 5.0, bc=0, algo=1, start=0, seed=7, nsweeps=3,
+             updates=(0, 1), normalize_at=(2,))
+
+
+def test_256cube_at_full_size(gpu, oracle_mod):
+    """256^3 (1.6 GB of links, the DRAM-bound lattice): one heat-bath +
+    over-relaxation step, all 16.8 M sites compared."""
+    run_pair(gpu, oracle_mod, "256^3", 256, 256, 256, 5.0, bc=0, algo=1, start=0, seed=7, nsweeps=1,
+             updates=(0, 1))
+
+
 @pytest.mark.parametrize("bc", [0, 1, 2])
 @pytest.mark.parametrize("algo", [0, 1])
 @pytest.mark.parametrize("prec", [0, 1])
