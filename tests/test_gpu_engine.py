@@ -206,3 +206,37 @@ def test_server_rejects_invalid_and_oversized(gpu):
         assert r.id == jid and r.status == 0
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_ensemble_of_64_at_full_size(gpu, prec):
+    """The measured ensemble itself (64 lattices, 1.89 M sites, Lt = Lx over
+    even 8..48, Ly over even 8..64, beta in [2, 8], every boundary, seeds
+    1000 + i, 20 + 80 sweeps) in one engine with graphs and the autotuned
+    shape: every job's 80 energies and plaquettes equal, bit for bit, those
+    of the same job run alone through the C driver su2_run."""
+    import ctypes as C
+    import os
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+
+    q = gpu
+    _say(f"[engine] 64-lattice ensemble prec={prec}")
+    lib = q.load_library()
+    jobs = bench.ensemble_jobs(q, 64, 20, 80)
+    need = sum(lib.su2_job_bytes(C.byref(j.c()), prec) for j in jobs)
+    eng = q.Engine(precision=prec, max_jobs=64, budget_bytes=int(need * 1.05) + (16 << 20), use_graph=True)
+    try:
+        slots = [eng.add(j) for j in jobs]
+        eng.sync()
+        eng.autotune()
+        eng.run()
+        for s, j in zip(slots, jobs):
+            got = eng.read(s)
+            r = q.run(Lt=j.Lt, Lx=j.Lx, Ly=j.Ly, beta=j.beta, boundary=j.bc, algo=j.algo, init=q.START_HOT,
+                      N=j.n_meas, N_term=j.n_therm, seed=j.seed, precision=prec, save=1, sconf=1)
+            assert np.array_equal(got["E"].view(np.uint64), r.energy.view(np.uint64)), f"job {s}: energies"
+            assert np.array_equal(got["P"].view(np.uint64), r.plaquette.view(np.uint64)), f"job {s}: plaquettes"
+    finally:
+        eng.close()
