@@ -72,6 +72,41 @@ def test_c1_run_matches_checker_and_writes_reference_files(gpu, oracle_mod, tmp_
     assert ac[2] == f"1 {rho[0]:.10f}"
 
 
+@pytest.mark.parametrize("bc", [1, 2])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_c3_run_matches_checker(gpu, oracle_mod, bc, prec):
+    """The full 32x32x48 PCC / PMC Metropolis run (eps 0.5, 4 hits, beta
+    4.0, hot start, seed 4321, 100 + 100 sweeps) through su2_run: every
+    measurement against the checker's run loop, FP64 and FP32 links, and the
+    acceptance against the checker's counters."""
+    q = gpu
+    print(f"[driver] C3 32x32x48 bc={bc} prec={prec}", flush=True)
+    r = q.run(Lt=32, Lx=32, Ly=48, beta=4.0, boundary=bc, algo=0, init=q.START_HOT, N=100, N_term=100, seed=4321,
+              epsilon=0.5, upd=4, precision=prec)
+    o = oracle_mod.Oracle(32, 32, 48, 4.0, 0.5, 4, f32=(prec == 1))
+    o.setup_kernel(4321)
+    o.start(0, bc)
+    for i in range(1, 101):
+        if i % 100 == 0:
+            o.normalize()
+        o.mc_up(bc, 0, 0)
+    o.normalize()
+    o.accept[:] = 0
+    E, P = [], []
+    for i in range(1, 101):
+        o.mc_up(bc, 0, 0)
+        if i % 100 == 0:
+            o.normalize()
+        e, p = o.en_av(bc == 2)
+        E.append(e)
+        P.append(p)
+    assert close(r.energy, np.array(E)) and close(r.plaquette, np.array(P))
+    acc = o.accept.astype(np.float64).sum() / o.V / (100 * 3 * 4)
+    if bc == 1:
+        acc /= (47 / 48 + 1 / 48 / 3)
+    assert abs(r.accept - acc) <= 1e-12
+
+
 def test_resume_from_checkpoint_is_bit_exact(gpu, tmp_path):
     q = gpu
     print("[driver] checkpoint/resume", flush=True)
