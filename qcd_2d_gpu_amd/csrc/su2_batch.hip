@@ -109,7 +109,10 @@ template <int MU, int BC, int ALGO, typename LinkT>
 __device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t h, int in_range, su2_kp_queue &q)
 {
     if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
-        su2_sweep_site_hb_wg<MU, BC, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, in_range, q);
+        /* early RNG load for FP32 links only (measured: batched FP32 -4.5 %,
+         * FP64 +1.6 % per ensemble run; see su2_sweep_site_hb_wg) */
+        su2_sweep_site_hb_wg<MU, BC, LinkT, sizeof(LinkT) == sizeof(su2_l32)>((LinkT *)d.U, d.rng, d.accept, d.g,
+                                                                              d.ph, parity, h, in_range, q);
     else if (in_range)
         su2_sweep_site<MU, BC, SU2_K_METRO, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
 }

@@ -76,7 +76,9 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
     const uint32_t h = sweep_block(order, strip, bpp) * blockDim.x + threadIdx.x;
     if constexpr (UPD == SU2_K_HB) {
         __shared__ su2_kp_queue q;
-        su2_sweep_site_hb_wg<MU, BC, LinkT>(U, rng, accept, g, ph, parity, h, h < g.Vh, q);
+        /* FP64 links: early RNG load (see su2_sweep_site_hb_wg) */
+        su2_sweep_site_hb_wg<MU, BC, LinkT, sizeof(LinkT) == sizeof(su2_l64)>(U, rng, accept, g, ph, parity, h,
+                                                                              h < g.Vh, q);
     } else {
         if (h >= g.Vh) return;
         su2_sweep_site<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, h);

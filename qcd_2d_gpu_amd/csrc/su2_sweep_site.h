@@ -192,7 +192,7 @@ struct su2_kp_queue {
     float lf[256];
 };
 
-template <int MU, int BC, typename LinkT>
+template <int MU, int BC, typename LinkT, bool EARLY_RNG = true>
 __device__ static inline __attribute__((always_inline)) void
 su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept,
                      const su2_kgeom &g, const su2_kphys &ph, int parity, uint32_t h, int in_range,
@@ -200,15 +200,34 @@ su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__r
 {
     typedef su2_link_io<LinkT> io;
 
+    /* EARLY_RNG: the stream's state is loaded before the staple, so that
+     * its latency overlaps the neighbour loads; loaded after it, each
+     * wavefront waits for one more memory round trip.  It keeps the six
+     * words live across the staple.  Measured, alternating on one box: the
+     * FP64-link single-lattice sweep (capped at 4 waves per SIMD anyway,
+     * 80 -> 92 VGPRs) gains 2-4 % per launch at 128^3; the uncapped
+     * FP32-link one would go from 74 to 82 VGPRs, 6 to 5 waves, and lose
+     * 3 %; the batched sweep gains 4.5 % with FP32 links and loses 1.6 %
+     * with FP64 links.  The callers choose accordingly.  (Issuing all
+     * twelve neighbour loads before the first product gained nothing more
+     * and cost FP32-link over-relaxation 5 %.) */
+    uint32_t s[6];
+    if constexpr (EARLY_RNG) {
+        if (in_range) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * g.Vh + h];
+        }
+    }
     su2_q A = su2_q_zero();
     const int active = in_range ? su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A) : 0;
     const size_t Vh = g.Vh;
-    uint32_t s[6];
     float inv = 0.0f, lf = 0.0f;
     int done = 1;
     if (active) {
+        if constexpr (!EARLY_RNG) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * Vh + h];
+            for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * Vh + h];
+        }
         inv = su2_kp_setup(&A, ph.beta);
         done = su2_kp_trial(s, inv, &lf);
     }
