@@ -28,6 +28,14 @@
  * is su2_fmath.h (project-owned log/exp/sinpi/cospi built from IEEE basic
  * operations), which is pinned separately against high-precision values.
  *
+ * Rounding: every product that feeds a sum is fused, in one fixed pattern:
+ * the first product of a sum rounded on its own, each later term added by
+ * fma()/fmaf() in the reference's term order (the contraction a CUDA
+ * compiler applies to these expressions by default).  No other contraction
+ * happens (-ffp-contract=off); -mfma makes the fused operations single
+ * instructions, and without it the C library's correctly rounded fma()
+ * gives the same values.
+ *
  * FP32-link mode (an extension of the build): every store into the field is
  * rounded to float; all arithmetic stays in double.
  */
@@ -203,7 +211,10 @@ void orc_setup_kernel(orc_rng *state, int nstates, uint64_t seed)
 /* ------------------------------------------------------------------------
  * SU(2) algebra, functions/su2.h
  * ---------------------------------------------------------------------- */
-double su2_det(const double *A) { return A[0] * A[0] + A[1] * A[1] + A[2] * A[2] + A[3] * A[3]; } /* su2.h:1-3 */
+double su2_det(const double *A) /* su2.h:1-3 */
+{
+    return fma(A[3], A[3], fma(A[2], A[2], fma(A[1], A[1], A[0] * A[0])));
+}
 
 void su2_normalize(double *A) /* su2.h:4-8 */
 {
@@ -237,10 +248,10 @@ void su2_zero(double *A)
 
 void su2_product(const double *A, const double *B, double *C) /* su2.h:50-55: C = A B */
 {
-    double c0 = A[0] * B[0] - A[1] * B[1] - A[2] * B[2] - A[3] * B[3];
-    double c1 = A[1] * B[0] + A[0] * B[1] + A[2] * B[3] - A[3] * B[2];
-    double c2 = A[2] * B[0] + A[3] * B[1] + A[0] * B[2] - A[1] * B[3];
-    double c3 = A[3] * B[0] - A[2] * B[1] + A[0] * B[3] + A[1] * B[2];
+    double c0 = fma(-A[3], B[3], fma(-A[2], B[2], fma(-A[1], B[1], A[0] * B[0])));
+    double c1 = fma(-A[3], B[2], fma(A[2], B[3], fma(A[0], B[1], A[1] * B[0])));
+    double c2 = fma(-A[1], B[3], fma(A[0], B[2], fma(A[3], B[1], A[2] * B[0])));
+    double c3 = fma(A[1], B[2], fma(A[0], B[3], fma(-A[2], B[1], A[3] * B[0])));
     C[0] = c0;
     C[1] = c1;
     C[2] = c2;
@@ -249,10 +260,10 @@ void su2_product(const double *A, const double *B, double *C) /* su2.h:50-55: C 
 
 void su2_product_tc(const double *A, const double *B, double *C) /* su2.h:56-61: C = A B^dag */
 {
-    double c0 = A[0] * B[0] + A[1] * B[1] + A[2] * B[2] + A[3] * B[3];
-    double c1 = A[1] * B[0] - A[0] * B[1] - A[2] * B[3] + A[3] * B[2];
-    double c2 = A[2] * B[0] - A[3] * B[1] - A[0] * B[2] + A[1] * B[3];
-    double c3 = A[3] * B[0] + A[2] * B[1] - A[0] * B[3] - A[1] * B[2];
+    double c0 = fma(A[3], B[3], fma(A[2], B[2], fma(A[1], B[1], A[0] * B[0])));
+    double c1 = fma(A[3], B[2], fma(-A[2], B[3], fma(-A[0], B[1], A[1] * B[0])));
+    double c2 = fma(A[1], B[3], fma(-A[0], B[2], fma(-A[3], B[1], A[2] * B[0])));
+    double c3 = fma(-A[1], B[2], fma(-A[0], B[3], fma(A[2], B[1], A[3] * B[0])));
     C[0] = c0;
     C[1] = c1;
     C[2] = c2;
@@ -261,10 +272,10 @@ void su2_product_tc(const double *A, const double *B, double *C) /* su2.h:56-61:
 
 void su2_product_tc1(const double *A, const double *B, double *C) /* su2.h:62-67: C = A^dag B */
 {
-    double c0 = A[0] * B[0] + A[1] * B[1] + A[2] * B[2] + A[3] * B[3];
-    double c1 = -A[1] * B[0] + A[0] * B[1] - A[2] * B[3] + A[3] * B[2];
-    double c2 = -A[2] * B[0] - A[3] * B[1] + A[0] * B[2] + A[1] * B[3];
-    double c3 = -A[3] * B[0] + A[2] * B[1] + A[0] * B[3] - A[1] * B[2];
+    double c0 = fma(A[3], B[3], fma(A[2], B[2], fma(A[1], B[1], A[0] * B[0])));
+    double c1 = fma(A[3], B[2], fma(-A[2], B[3], fma(A[0], B[1], -A[1] * B[0])));
+    double c2 = fma(A[1], B[3], fma(A[0], B[2], fma(-A[3], B[1], -A[2] * B[0])));
+    double c3 = fma(-A[1], B[2], fma(A[0], B[3], fma(A[2], B[1], -A[3] * B[0])));
     C[0] = c0;
     C[1] = c1;
     C[2] = c2;
@@ -273,10 +284,10 @@ void su2_product_tc1(const double *A, const double *B, double *C) /* su2.h:62-67
 
 void su2_product_tc2(const double *A, const double *B, double *C) /* su2.h:68-73: C = A^dag B^dag */
 {
-    double c0 = A[0] * B[0] - A[1] * B[1] - A[2] * B[2] - A[3] * B[3];
-    double c1 = -A[1] * B[0] - A[0] * B[1] + A[2] * B[3] - A[3] * B[2];
-    double c2 = -A[2] * B[0] + A[3] * B[1] - A[0] * B[2] - A[1] * B[3];
-    double c3 = -A[3] * B[0] - A[2] * B[1] - A[0] * B[3] + A[1] * B[2];
+    double c0 = fma(-A[3], B[3], fma(-A[2], B[2], fma(-A[1], B[1], A[0] * B[0])));
+    double c1 = fma(-A[3], B[2], fma(A[2], B[3], fma(-A[0], B[1], -A[1] * B[0])));
+    double c2 = fma(-A[1], B[3], fma(-A[0], B[2], fma(A[3], B[1], -A[2] * B[0])));
+    double c3 = fma(A[1], B[2], fma(-A[0], B[3], fma(-A[2], B[1], -A[3] * B[0])));
     C[0] = c0;
     C[1] = c1;
     C[2] = c2;
@@ -285,7 +296,7 @@ void su2_product_tc2(const double *A, const double *B, double *C) /* su2.h:68-73
 
 double su2_prod_trace(const double *A, const double *B) /* su2.h:74-76 */
 {
-    return A[0] * B[0] - A[1] * B[1] - A[2] * B[2] - A[3] * B[3];
+    return fma(-A[3], B[3], fma(-A[2], B[2], fma(-A[1], B[1], A[0] * B[0])));
 }
 
 void su2_sum(double *A, const double *B) /* su2.h:77-80 */
@@ -302,9 +313,9 @@ void su2_update(const double *A, double *B, double ep, orc_rng *state) /* su2.h:
 {
     double r[4];
     for (int i = 1; i < 4; i++) r[i] = orc_uniform_double(state) - 0.5;
-    double n = sqrt(r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+    double n = sqrt(fma(r[3], r[3], fma(r[2], r[2], r[1] * r[1])));
     for (int i = 0; i < 3; i++) r[i + 1] = ep * r[i + 1] / n;
-    r[0] = sqrt(1 - ep * ep);
+    r[0] = sqrt(fma(-ep, ep, 1.0));
     su2_product(r, A, B);
 }
 
@@ -344,18 +355,18 @@ int orc_heat_bath(double *A, double *phi, double beta, orc_rng *state) /* mc_upd
     do {
         float r0 = orc_uniform(state), r1 = orc_uniform(state), r2 = orc_uniform(state);
         float c = su2_cospif(2 * r1);
-        lf = -(su2_logf(r0) + c * c * su2_logf(r2)) * inv;
+        lf = -fmaf(c * c, su2_logf(r2), su2_logf(r0)) * inv;
         aux = 1 - orc_uniform(state);
         /* a zero or non-finite staple (inv infinite or NaN) would be rejected
          * for ever, as in the reference; the loop ends after one trial */
     } while (aux * aux > 1 - lf && inv <= FLT_MAX);
     X[0] = 1 - 2 * (double)lf;
-    double n = sqrt(1 - X[0] * X[0]);
+    double n = sqrt(fma(-X[0], X[0], 1.0));
     float ct = 2 * orc_uniform(state) - 1, sp, cp;
     su2_sincospif(2 * orc_uniform(state), &sp, &cp);
-    float st = su2_sqrtf(su2_fmaxf(0.f, 1 - ct * ct));
+    float st = su2_sqrtf(su2_fmaxf(0.f, fmaf(-ct, ct, 1.f)));
     double v1 = ct, v2 = (double)st * cp, v3 = (double)st * sp;
-    double s = n * su2_rsqrt(v1 * v1 + v2 * v2 + v3 * v3);
+    double s = n * su2_rsqrt(fma(v3, v3, fma(v2, v2, v1 * v1)));
     X[1] = v1 * s;
     X[2] = v2 * s;
     X[3] = v3 * s;

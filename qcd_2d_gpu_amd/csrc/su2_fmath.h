@@ -11,10 +11,12 @@
  * device kernels and the CPU checker must evaluate them identically.  Vendor
  * libraries (glibc libm, the device math library) do not promise that, so
  * this header builds each function from IEEE-754 basic operations only
- * (+ - * / and sqrt, all correctly rounded on both targets) with a fixed
- * evaluation order.  Every file that includes it must be compiled with
- * floating-point contraction disabled (-ffp-contract=off); the pragma below
- * enforces that for clang/hipcc.
+ * (+ - * /, fused multiply-add and sqrt, all correctly rounded on both
+ * targets) with a fixed evaluation order; every polynomial step is one fused
+ * multiply-add.  Every file that includes it must be compiled with automatic
+ * floating-point contraction disabled (-ffp-contract=off), so that the only
+ * fused operations are the written ones; the pragma below enforces that for
+ * clang/hipcc.
  *
  * Methods (all from the mathematics; no vendor library internals):
  *  - log: x = 2^e * m, m in [sqrt(2)/2, sqrt(2)); log m = 2 atanh(f),
@@ -86,6 +88,10 @@ SU2_FM_FN double su2_fm_pow2(int k)
 
 SU2_FM_FN double su2_sqrt_rn(double x) { return __builtin_sqrt(x); }
 
+/* a * b + c with one rounding */
+SU2_FM_FN double su2_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+SU2_FM_FN float su2_fmaf(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 /* rsqrt of the reference (functions/su2.h:5,17; mc_updates.h:58) as two
  * correctly rounded operations. */
 SU2_FM_FN double su2_rsqrt(double x) { return 1.0 / __builtin_sqrt(x); }
@@ -112,15 +118,15 @@ SU2_FM_FN double su2_log_normal_d(double x)
     double f = (m - 1.0) / (m + 1.0);
     double s = f * f;
     double p = 1.0 / 15.0;
-    p = 1.0 / 13.0 + s * p;
-    p = 1.0 / 11.0 + s * p;
-    p = 1.0 / 9.0 + s * p;
-    p = 1.0 / 7.0 + s * p;
-    p = 1.0 / 5.0 + s * p;
-    p = 1.0 / 3.0 + s * p;
-    double logm = (2.0 * f) + (2.0 * f) * (s * p);
+    p = su2_fma(s, p, 1.0 / 13.0);
+    p = su2_fma(s, p, 1.0 / 11.0);
+    p = su2_fma(s, p, 1.0 / 9.0);
+    p = su2_fma(s, p, 1.0 / 7.0);
+    p = su2_fma(s, p, 1.0 / 5.0);
+    p = su2_fma(s, p, 1.0 / 3.0);
+    double logm = su2_fma(2.0 * f, s * p, 2.0 * f);
     double de = (double)e;
-    return de * SU2_LN2_HI + (de * SU2_LN2_LO + logm);
+    return su2_fma(de, SU2_LN2_HI, su2_fma(de, SU2_LN2_LO, logm));
 }
 
 /* Natural logarithm for finite x > 0 (subnormals scaled into range). */
@@ -141,23 +147,23 @@ SU2_FM_FN double su2_exp_d(double x)
     if (x != x) return x;
     if (x > 709.782712893384) return __builtin_inf();
     if (x < -745.1332191019412) return 0.0;
-    double kd = __builtin_floor(x * SU2_INV_LN2 + 0.5);
+    double kd = __builtin_floor(su2_fma(x, SU2_INV_LN2, 0.5));
     int k = (int)kd;
-    double r = (x - kd * SU2_LN2_HI) - kd * SU2_LN2_LO;
+    double r = su2_fma(-kd, SU2_LN2_LO, su2_fma(-kd, SU2_LN2_HI, x));
     double p = 1.0 / 6227020800.0;              /* 1/13! */
-    p = 1.0 / 479001600.0 + r * p;              /* 1/12! */
-    p = 1.0 / 39916800.0 + r * p;               /* 1/11! */
-    p = 1.0 / 3628800.0 + r * p;                /* 1/10! */
-    p = 1.0 / 362880.0 + r * p;                 /* 1/9! */
-    p = 1.0 / 40320.0 + r * p;                  /* 1/8! */
-    p = 1.0 / 5040.0 + r * p;                   /* 1/7! */
-    p = 1.0 / 720.0 + r * p;                    /* 1/6! */
-    p = 1.0 / 120.0 + r * p;                    /* 1/5! */
-    p = 1.0 / 24.0 + r * p;                     /* 1/4! */
-    p = 1.0 / 6.0 + r * p;                      /* 1/3! */
-    p = 0.5 + r * p;                            /* 1/2! */
-    p = 1.0 + r * p;
-    p = 1.0 + r * p;
+    p = su2_fma(r, p, 1.0 / 479001600.0);       /* 1/12! */
+    p = su2_fma(r, p, 1.0 / 39916800.0);        /* 1/11! */
+    p = su2_fma(r, p, 1.0 / 3628800.0);         /* 1/10! */
+    p = su2_fma(r, p, 1.0 / 362880.0);          /* 1/9! */
+    p = su2_fma(r, p, 1.0 / 40320.0);           /* 1/8! */
+    p = su2_fma(r, p, 1.0 / 5040.0);            /* 1/7! */
+    p = su2_fma(r, p, 1.0 / 720.0);             /* 1/6! */
+    p = su2_fma(r, p, 1.0 / 120.0);             /* 1/5! */
+    p = su2_fma(r, p, 1.0 / 24.0);              /* 1/4! */
+    p = su2_fma(r, p, 1.0 / 6.0);               /* 1/3! */
+    p = su2_fma(r, p, 0.5);                     /* 1/2! */
+    p = su2_fma(r, p, 1.0);
+    p = su2_fma(r, p, 1.0);
     if (k > 1023) return (p * 2.0) * su2_fm_pow2(k - 1);
     if (k < -1021) return (p * su2_fm_pow2(k + 600)) * 0x1p-600;
     return p * su2_fm_pow2(k);
@@ -176,14 +182,14 @@ SU2_FM_FN double su2_fm_sinpi_k(double d)
     double t = d * SU2_PI;
     double z = t * t;
     double p = 1.0 / 355687428096000.0;         /*  1/17! */
-    p = -1.0 / 1307674368000.0 + z * p;         /* -1/15! */
-    p = 1.0 / 6227020800.0 + z * p;             /*  1/13! */
-    p = -1.0 / 39916800.0 + z * p;              /* -1/11! */
-    p = 1.0 / 362880.0 + z * p;                 /*  1/9!  */
-    p = -1.0 / 5040.0 + z * p;                  /* -1/7!  */
-    p = 1.0 / 120.0 + z * p;                    /*  1/5!  */
-    p = -1.0 / 6.0 + z * p;                     /* -1/3!  */
-    return t + (t * z) * p;
+    p = su2_fma(z, p, -1.0 / 1307674368000.0);  /* -1/15! */
+    p = su2_fma(z, p, 1.0 / 6227020800.0);      /*  1/13! */
+    p = su2_fma(z, p, -1.0 / 39916800.0);       /* -1/11! */
+    p = su2_fma(z, p, 1.0 / 362880.0);          /*  1/9!  */
+    p = su2_fma(z, p, -1.0 / 5040.0);           /* -1/7!  */
+    p = su2_fma(z, p, 1.0 / 120.0);             /*  1/5!  */
+    p = su2_fma(z, p, -1.0 / 6.0);              /* -1/3!  */
+    return su2_fma(t * z, p, t);
 }
 
 /* Exact reduction: r = a - 2 round(a/2) in [-1, 1); returns q = |r|. */

@@ -4,9 +4,15 @@
  * A link is four reals a0..a3 standing for the matrix
  *     [[a0 + i a1,  a2 + i a3], [-a2 + i a3,  a0 - i a1]]
  * (functions/su2.h:98-100).  The products below are the 2x2 matrix products
- * in that basis.  Their term order follows functions/su2.h:50-76 exactly:
- * with contraction disabled that fixes every rounding, which is what lets
- * the device kernels agree bit for bit with the CPU checker.
+ * in that basis.  Their term order follows functions/su2.h:50-76 exactly,
+ * and each four-term sum is contracted in one fixed pattern: the first
+ * product rounded on its own, every later term added by a fused
+ * multiply-add, t = fma(+-a_k, b_k, t) (the contraction a CUDA compiler
+ * applies to these expressions by default; fma is a correctly rounded
+ * IEEE-754 operation on the host and on gfx950 alike).  With automatic
+ * contraction disabled everywhere else, that fixes every rounding, which is
+ * what lets the device kernels agree bit for bit with the CPU checker.
+ * Fusing cuts each product from 28 to 16 FP64 instructions.
  */
 #ifndef SU2_QUAT_H
 #define SU2_QUAT_H
@@ -22,6 +28,9 @@
 typedef struct su2_q {
     double a0, a1, a2, a3;
 } su2_q;
+
+/* fused multiply-add, one rounding */
+#define SU2_FMA(a, b, c) __builtin_fma((a), (b), (c))
 
 SU2_Q_FN su2_q su2_q_make(double a0, double a1, double a2, double a3)
 {
@@ -39,49 +48,49 @@ SU2_Q_FN su2_q su2_q_id(void) { return su2_q_make(1.0, 0.0, 0.0, 0.0); }
 /* A B  (functions/su2.h:50-55) */
 SU2_Q_FN su2_q su2_q_mul(su2_q A, su2_q B)
 {
-    return su2_q_make(A.a0 * B.a0 - A.a1 * B.a1 - A.a2 * B.a2 - A.a3 * B.a3,
-                      A.a1 * B.a0 + A.a0 * B.a1 + A.a2 * B.a3 - A.a3 * B.a2,
-                      A.a2 * B.a0 + A.a3 * B.a1 + A.a0 * B.a2 - A.a1 * B.a3,
-                      A.a3 * B.a0 - A.a2 * B.a1 + A.a0 * B.a3 + A.a1 * B.a2);
+    return su2_q_make(SU2_FMA(-A.a3, B.a3, SU2_FMA(-A.a2, B.a2, SU2_FMA(-A.a1, B.a1, A.a0 * B.a0))),
+                      SU2_FMA(-A.a3, B.a2, SU2_FMA(A.a2, B.a3, SU2_FMA(A.a0, B.a1, A.a1 * B.a0))),
+                      SU2_FMA(-A.a1, B.a3, SU2_FMA(A.a0, B.a2, SU2_FMA(A.a3, B.a1, A.a2 * B.a0))),
+                      SU2_FMA(A.a1, B.a2, SU2_FMA(A.a0, B.a3, SU2_FMA(-A.a2, B.a1, A.a3 * B.a0))));
 }
 
 /* A B^dagger  (functions/su2.h:56-61) */
 SU2_Q_FN su2_q su2_q_mul_nd(su2_q A, su2_q B)
 {
-    return su2_q_make(A.a0 * B.a0 + A.a1 * B.a1 + A.a2 * B.a2 + A.a3 * B.a3,
-                      A.a1 * B.a0 - A.a0 * B.a1 - A.a2 * B.a3 + A.a3 * B.a2,
-                      A.a2 * B.a0 - A.a3 * B.a1 - A.a0 * B.a2 + A.a1 * B.a3,
-                      A.a3 * B.a0 + A.a2 * B.a1 - A.a0 * B.a3 - A.a1 * B.a2);
+    return su2_q_make(SU2_FMA(A.a3, B.a3, SU2_FMA(A.a2, B.a2, SU2_FMA(A.a1, B.a1, A.a0 * B.a0))),
+                      SU2_FMA(A.a3, B.a2, SU2_FMA(-A.a2, B.a3, SU2_FMA(-A.a0, B.a1, A.a1 * B.a0))),
+                      SU2_FMA(A.a1, B.a3, SU2_FMA(-A.a0, B.a2, SU2_FMA(-A.a3, B.a1, A.a2 * B.a0))),
+                      SU2_FMA(-A.a1, B.a2, SU2_FMA(-A.a0, B.a3, SU2_FMA(A.a2, B.a1, A.a3 * B.a0))));
 }
 
 /* A^dagger B  (functions/su2.h:62-67) */
 SU2_Q_FN su2_q su2_q_mul_dn(su2_q A, su2_q B)
 {
-    return su2_q_make(A.a0 * B.a0 + A.a1 * B.a1 + A.a2 * B.a2 + A.a3 * B.a3,
-                      -A.a1 * B.a0 + A.a0 * B.a1 - A.a2 * B.a3 + A.a3 * B.a2,
-                      -A.a2 * B.a0 - A.a3 * B.a1 + A.a0 * B.a2 + A.a1 * B.a3,
-                      -A.a3 * B.a0 + A.a2 * B.a1 + A.a0 * B.a3 - A.a1 * B.a2);
+    return su2_q_make(SU2_FMA(A.a3, B.a3, SU2_FMA(A.a2, B.a2, SU2_FMA(A.a1, B.a1, A.a0 * B.a0))),
+                      SU2_FMA(A.a3, B.a2, SU2_FMA(-A.a2, B.a3, SU2_FMA(A.a0, B.a1, -A.a1 * B.a0))),
+                      SU2_FMA(A.a1, B.a3, SU2_FMA(A.a0, B.a2, SU2_FMA(-A.a3, B.a1, -A.a2 * B.a0))),
+                      SU2_FMA(-A.a1, B.a2, SU2_FMA(A.a0, B.a3, SU2_FMA(A.a2, B.a1, -A.a3 * B.a0))));
 }
 
 /* A^dagger B^dagger  (functions/su2.h:68-73) */
 SU2_Q_FN su2_q su2_q_mul_dd(su2_q A, su2_q B)
 {
-    return su2_q_make(A.a0 * B.a0 - A.a1 * B.a1 - A.a2 * B.a2 - A.a3 * B.a3,
-                      -A.a1 * B.a0 - A.a0 * B.a1 + A.a2 * B.a3 - A.a3 * B.a2,
-                      -A.a2 * B.a0 + A.a3 * B.a1 - A.a0 * B.a2 - A.a1 * B.a3,
-                      -A.a3 * B.a0 - A.a2 * B.a1 - A.a0 * B.a3 + A.a1 * B.a2);
+    return su2_q_make(SU2_FMA(-A.a3, B.a3, SU2_FMA(-A.a2, B.a2, SU2_FMA(-A.a1, B.a1, A.a0 * B.a0))),
+                      SU2_FMA(-A.a3, B.a2, SU2_FMA(A.a2, B.a3, SU2_FMA(-A.a0, B.a1, -A.a1 * B.a0))),
+                      SU2_FMA(-A.a1, B.a3, SU2_FMA(-A.a0, B.a2, SU2_FMA(A.a3, B.a1, -A.a2 * B.a0))),
+                      SU2_FMA(A.a1, B.a2, SU2_FMA(-A.a0, B.a3, SU2_FMA(-A.a2, B.a1, -A.a3 * B.a0))));
 }
 
 /* First component of A B^dagger: 1/2 Re tr(A B^dagger). */
 SU2_Q_FN double su2_q_mul_nd_re(su2_q A, su2_q B)
 {
-    return A.a0 * B.a0 + A.a1 * B.a1 + A.a2 * B.a2 + A.a3 * B.a3;
+    return SU2_FMA(A.a3, B.a3, SU2_FMA(A.a2, B.a2, SU2_FMA(A.a1, B.a1, A.a0 * B.a0)));
 }
 
 /* 1/2 Re tr(A B)  (functions/su2.h:74-76) */
 SU2_Q_FN double su2_q_retr(su2_q A, su2_q B)
 {
-    return A.a0 * B.a0 - A.a1 * B.a1 - A.a2 * B.a2 - A.a3 * B.a3;
+    return SU2_FMA(-A.a3, B.a3, SU2_FMA(-A.a2, B.a2, SU2_FMA(-A.a1, B.a1, A.a0 * B.a0)));
 }
 
 /* A + B, component-wise (functions/su2.h:77-80) */
@@ -104,7 +113,7 @@ SU2_Q_FN su2_q su2_q_scale(su2_q A, double s)
 /* |a|^2 = det  (functions/su2.h:1-3) */
 SU2_Q_FN double su2_q_det(su2_q A)
 {
-    return A.a0 * A.a0 + A.a1 * A.a1 + A.a2 * A.a2 + A.a3 * A.a3;
+    return SU2_FMA(A.a3, A.a3, SU2_FMA(A.a2, A.a2, SU2_FMA(A.a1, A.a1, A.a0 * A.a0)));
 }
 
 /* a <- a / |a|  (functions/su2.h:4-8) */

@@ -25,6 +25,11 @@
 #endif
 
 /*
+ * Every product that feeds a sum below is fused (one fma, written out), as
+ * in the quaternion algebra: the contraction a CUDA compiler applies to the
+ * reference's expressions by default, fixed here so that the host and the
+ * device round alike.
+ *
  * Kennedy-Pendleton heat-bath (functions/mc_updates.h:43-64).
  * A is normalised to A^ with norm a; X0 = 1 - 2 lambda is drawn from
  * sqrt(1 - X0^2) exp(beta a X0) by the rejection loop, evaluated in float as
@@ -54,7 +59,7 @@ SU2_U_FN int su2_kp_trial(uint32_t *s, float inv, float *lf)
     float r1 = su2_mrg_uniform(s);
     float r2 = su2_mrg_uniform(s);
     float c = su2_cospif(2.0f * r1);
-    float l = -(su2_logf(r0) + c * c * su2_logf(r2)) * inv;
+    float l = -su2_fmaf(c * c, su2_logf(r2), su2_logf(r0)) * inv;
     float aux = 1.0f - su2_mrg_uniform(s);
     *lf = l;
     return !(aux * aux > 1.0f - l) || !(inv <= FLT_MAX);
@@ -64,13 +69,13 @@ SU2_U_FN int su2_kp_trial(uint32_t *s, float inv, float *lf)
 SU2_U_FN su2_q su2_kp_finish(su2_q A, float lf, uint32_t *s)
 {
     double x0 = 1.0 - 2.0 * (double)lf;
-    double n = su2_sqrt_rn(1.0 - x0 * x0);
+    double n = su2_sqrt_rn(su2_fma(-x0, x0, 1.0));
     float ct = 2.0f * su2_mrg_uniform(s) - 1.0f;
     float sp, cp;
     su2_sincospif(2.0f * su2_mrg_uniform(s), &sp, &cp);
-    float st = su2_sqrtf(su2_fmaxf(0.0f, 1.0f - ct * ct));
+    float st = su2_sqrtf(su2_fmaxf(0.0f, su2_fmaf(-ct, ct, 1.0f)));
     double v1 = ct, v2 = (double)st * cp, v3 = (double)st * sp;
-    double sc = n * su2_rsqrt(v1 * v1 + v2 * v2 + v3 * v3);
+    double sc = n * su2_rsqrt(su2_fma(v3, v3, su2_fma(v2, v2, v1 * v1)));
     su2_q X = su2_q_make(x0, v1 * sc, v2 * sc, v3 * sc);
     return su2_q_mul_nd(X, A);
 }
@@ -112,11 +117,11 @@ SU2_U_FN int su2_metropolis(su2_q A, su2_q *U, double beta, double eps, int hits
         double r1 = su2_mrg_uniform_double(s) - 0.5;
         double r2 = su2_mrg_uniform_double(s) - 0.5;
         double r3 = su2_mrg_uniform_double(s) - 0.5;
-        double n = su2_sqrt_rn(r1 * r1 + r2 * r2 + r3 * r3);
+        double n = su2_sqrt_rn(su2_fma(r3, r3, su2_fma(r2, r2, r1 * r1)));
         r1 = eps * r1 / n;
         r2 = eps * r2 / n;
         r3 = eps * r3 / n;
-        double r0 = su2_sqrt_rn(1.0 - eps * eps);
+        double r0 = su2_sqrt_rn(su2_fma(-eps, eps, 1.0));
         su2_q Unew = su2_q_mul(su2_q_make(r0, r1, r2, r3), *U);
         su2_q dU = su2_q_sub(Unew, *U);
         double dS = -beta * su2_q_retr(dU, A);

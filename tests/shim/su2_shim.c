@@ -61,3 +61,14 @@ void shim_q_mul(const double *A, const double *B, double *C, int kind)
     }
     st(C, c);
 }
+
+/* kind 0: det A, 1: 1/2 Re tr(A B), 2: first component of A B^dag */
+double shim_q_scalar(const double *A, const double *B, int kind)
+{
+    su2_q a = ld(A), b = ld(B);
+    switch (kind) {
+    case 0: return su2_q_det(a);
+    case 1: return su2_q_retr(a, b);
+    default: return su2_q_mul_nd_re(a, b);
+    }
+}

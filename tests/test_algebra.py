@@ -44,3 +44,79 @@ def test_quaternion_matrices_are_su2_times_norm():
     M = N.quat_to_mat(A)
     np.testing.assert_allclose(M @ N.dag(M), np.broadcast_to(np.eye(2), M.shape), atol=1e-14)
     np.testing.assert_allclose(np.linalg.det(M), 1.0, atol=1e-14)
+
+
+# Term lists of the four products in the reference's order (functions/su2.h:
+# 50-73): per component, (sign, index into A, index into B).
+_TERMS = {
+    "su2_product": [[(1, 0, 0), (-1, 1, 1), (-1, 2, 2), (-1, 3, 3)],
+                    [(1, 1, 0), (1, 0, 1), (1, 2, 3), (-1, 3, 2)],
+                    [(1, 2, 0), (1, 3, 1), (1, 0, 2), (-1, 1, 3)],
+                    [(1, 3, 0), (-1, 2, 1), (1, 0, 3), (1, 1, 2)]],
+    "su2_product_tc": [[(1, 0, 0), (1, 1, 1), (1, 2, 2), (1, 3, 3)],
+                       [(1, 1, 0), (-1, 0, 1), (-1, 2, 3), (1, 3, 2)],
+                       [(1, 2, 0), (-1, 3, 1), (-1, 0, 2), (1, 1, 3)],
+                       [(1, 3, 0), (1, 2, 1), (-1, 0, 3), (-1, 1, 2)]],
+    "su2_product_tc1": [[(1, 0, 0), (1, 1, 1), (1, 2, 2), (1, 3, 3)],
+                        [(-1, 1, 0), (1, 0, 1), (-1, 2, 3), (1, 3, 2)],
+                        [(-1, 2, 0), (-1, 3, 1), (1, 0, 2), (1, 1, 3)],
+                        [(-1, 3, 0), (1, 2, 1), (1, 0, 3), (-1, 1, 2)]],
+    "su2_product_tc2": [[(1, 0, 0), (-1, 1, 1), (-1, 2, 2), (-1, 3, 3)],
+                        [(-1, 1, 0), (-1, 0, 1), (1, 2, 3), (-1, 3, 2)],
+                        [(-1, 2, 0), (1, 3, 1), (-1, 0, 2), (-1, 1, 3)],
+                        [(-1, 3, 0), (-1, 2, 1), (-1, 0, 3), (1, 1, 2)]],
+}
+
+
+def _fused_sum(terms, a, b):
+    """First product rounded alone, every later term added with one rounding
+    (an exact fused multiply-add, computed with rationals)."""
+    from fractions import Fraction as F
+
+    s0, i0, j0 = terms[0]
+    t = (s0 * a[i0]) * b[j0]
+    for s, i, j in terms[1:]:
+        t = float(F(s * a[i]) * F(b[j]) + F(t))
+    return t
+
+
+def _plain_sum(terms, a, b):
+    t = 0.0
+    for k, (s, i, j) in enumerate(terms):
+        p = (s * a[i]) * b[j]
+        t = p if k == 0 else t + p
+    return t
+
+
+def test_products_round_as_the_fused_pattern(oracle_mod):
+    """The checker's and the product header's products, det, 1/2 Re tr(A B)
+    and the first component of A B^dag equal, bit for bit, an exact-rational
+    model of the documented rounding (first product alone, then one fused
+    multiply-add per term, in the reference's term order).  The unfused
+    evaluation differs on these inputs, so a lost fusion would be caught."""
+    lib = oracle_mod.load()
+    shim = _shim.load()
+    rng = np.random.default_rng(3)
+    A, B = _rand(rng, 300), _rand(rng, 300)
+    differs = 0
+    for kind, (name, terms) in enumerate(_TERMS.items()):
+        for i in range(len(A)):
+            a, b = np.ascontiguousarray(A[i]), np.ascontiguousarray(B[i])
+            want = np.array([_fused_sum(t, a, b) for t in terms])
+            c = np.zeros(4)
+            getattr(lib, name)(a.ctypes.data, b.ctypes.data, c.ctypes.data)
+            assert np.array_equal(c.view(np.uint64), want.view(np.uint64)), (name, i)
+            c2 = np.zeros(4)
+            shim.shim_q_mul(a.ctypes.data, b.ctypes.data, c2.ctypes.data, kind)
+            assert np.array_equal(c2.view(np.uint64), want.view(np.uint64)), (name, i)
+            differs += any(_plain_sum(t, a, b) != w for t, w in zip(terms, want))
+    assert differs > 100
+    for i in range(100):
+        a, b = np.ascontiguousarray(A[i]), np.ascontiguousarray(B[i])
+        det = _fused_sum([(1, k, k) for k in range(4)], a, a)
+        retr = _fused_sum(_TERMS["su2_product"][0], a, b)
+        nd_re = _fused_sum(_TERMS["su2_product_tc"][0], a, b)
+        assert lib.su2_det(a.ctypes.data) == det == shim.shim_q_scalar(a.ctypes.data, b.ctypes.data, 0)
+        assert lib.su2_prod_trace(a.ctypes.data, b.ctypes.data) == retr == shim.shim_q_scalar(
+            a.ctypes.data, b.ctypes.data, 1)
+        assert nd_re == shim.shim_q_scalar(a.ctypes.data, b.ctypes.data, 2)

@@ -5,13 +5,15 @@ at sampled half-sites, including the last one:
 - setup_kernel: the six RNG words equal the checker's MRG32k3a stream h;
 - the hot start (functions/su2.h:29-34): each link is the normalised vector
   of four uniform_double - 0.5 draws of stream h, taken for parity 0 then 1
-  and mu = y, x, t; recomputed here in IEEE double and compared bit for bit;
+  and mu = y, x, t; recomputed here in IEEE double (the squared norm summed
+  with fused multiply-adds, as the kernels do) and compared bit for bit;
 - one heat-bath sweep: every sampled site counts 3 accepted updates per
   parity, the links stay unitary after normalize, and en_av is deterministic.
 """
 import ctypes as C
 import math
 import sys
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -34,9 +36,14 @@ def _read(lib, base, offset, nbytes, dtype):
     return out
 
 
+def _fma(a, b, c):
+    """a * b + c rounded once (exact rationals)."""
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
 def _hot_link(r):
     a = [r.uniform_double() - 0.5 for _ in range(4)]
-    det = a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3]
+    det = _fma(a[3], a[3], _fma(a[2], a[2], _fma(a[1], a[1], a[0] * a[0])))
     inv = 1.0 / math.sqrt(det)
     return np.array([x * inv for x in a])
 
