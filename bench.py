@@ -121,66 +121,67 @@ def gpu_bench(args):
     total_ms = float(ms.value)
     meas = res.download(np.float64, 2 * args.steps, stream).reshape(args.steps, 2)
 
-    # roofline pass: each heat-bath (and over-relaxation) launch bracketed by
-    # events on the stream it is launched on
-    nl = 6 * args.roofline_sweeps
+    # roofline pass: the six heat-bath launches of a sweep bracketed by one
+    # pair of HIP events on the stream they are launched on (per launch =
+    # bracket / 6, so the event records add little); the over-relaxation
+    # sweeps between them run unbracketed, as in the step
+    ns = args.roofline_sweeps
     evs = []
-    for _ in range(2 * nl + 2):
+    for _ in range(2 * ns + 2):
         e = C.c_void_p()
         L.check(lib.su2_event_create(C.byref(e)), "event")
         evs.append(e)
-    k = 0
-    hb_ms, or_ms = [], []
-    g.synchronize()
-    for s in range(args.roofline_sweeps):
-        for upd in (q.UPDATE_LINK, q.UPDATE_OVERRELAX):
-            for mu in range(3):
-                for par in range(2):
-                    if upd == q.UPDATE_LINK:
-                        a, b = evs[2 * k], evs[2 * k + 1]
-                        L.check(lib.su2_event_record(a, stream), "record")
-                        g.mc_update(0, q.ALGO_HEATBATH, upd, mu, par)
-                        L.check(lib.su2_event_record(b, stream), "record")
-                        k += 1
-                    else:
-                        g.mc_update(0, q.ALGO_HEATBATH, upd, mu, par)
-    g.synchronize()
-    for i in range(k):
-        L.check(lib.su2_event_elapsed_ms(evs[2 * i], evs[2 * i + 1], C.byref(ms)), "elapsed")
-        hb_ms.append(float(ms.value))
-    # over-relaxation launches timed the same way
-    k2 = 0
-    for s in range(args.roofline_sweeps):
-        for mu in range(3):
-            for par in range(2):
-                a, b = evs[2 * k2], evs[2 * k2 + 1]
-                L.check(lib.su2_event_record(a, stream), "record")
-                g.mc_update(0, q.ALGO_HEATBATH, q.UPDATE_OVERRELAX, mu, par)
-                L.check(lib.su2_event_record(b, stream), "record")
-                k2 += 1
-    g.synchronize()
-    for i in range(k2):
-        L.check(lib.su2_event_elapsed_ms(evs[2 * i], evs[2 * i + 1], C.byref(ms)), "elapsed")
-        or_ms.append(float(ms.value))
-    # en_av (partial + final) and Metropolis (eps 0.5, 4 hits) launches, for
-    # the roofline of each kernel kind
-    def timed(fn, n):
-        out = []
-        for i in range(n):
-            a, b = evs[2 * i], evs[2 * i + 1]
-            L.check(lib.su2_event_record(a, stream), "record")
-            fn(i)
-            L.check(lib.su2_event_record(b, stream), "record")
+
+    def bracketed(fn, n, ends_itself=False):
+        """fn(i) for i < n, each call bracketed by an event pair (fn records
+        the closing event itself when ends_itself); returns the bracket
+        durations in ms."""
         g.synchronize()
+        for i in range(n):
+            L.check(lib.su2_event_record(evs[2 * i], stream), "record")
+            fn(i)
+            if not ends_itself:
+                L.check(lib.su2_event_record(evs[2 * i + 1], stream), "record")
+        g.synchronize()
+        out = []
         for i in range(n):
             L.check(lib.su2_event_elapsed_ms(evs[2 * i], evs[2 * i + 1], C.byref(ms)), "elapsed")
             out.append(float(ms.value))
         return out
 
-    nk = min(len(evs) // 2, 6 * args.roofline_sweeps)
-    en_ms = timed(lambda i: g.en_av_into(False, res.value, res.value + 8), nk)
+    def sweep_pair(i):
+        for mu in range(3):
+            for par in range(2):
+                g.mc_update(0, q.ALGO_HEATBATH, q.UPDATE_LINK, mu, par)
+        L.check(lib.su2_event_record(evs[2 * i + 1], stream), "record")
+        for mu in range(3):
+            for par in range(2):
+                g.mc_update(0, q.ALGO_HEATBATH, q.UPDATE_OVERRELAX, mu, par)
+
+    hb_ms = [t / 6 for t in bracketed(sweep_pair, ns, ends_itself=True)]
+
+    def or_sweep(i):
+        for mu in range(3):
+            for par in range(2):
+                g.mc_update(0, q.ALGO_HEATBATH, q.UPDATE_OVERRELAX, mu, par)
+
+    or_ms = [t / 6 for t in bracketed(or_sweep, ns)]
+
+    # en_av (partial + final; six measurements per bracket) and Metropolis
+    # (eps 0.5, 4 hits; one sweep of six launches per bracket)
+    def en_av6(i):
+        for _ in range(6):
+            g.en_av_into(False, res.value, res.value + 8)
+
+    en_ms = [t / 6 for t in bracketed(en_av6, ns)]
     g.set_params(epsilon=0.5, updates=4)
-    me_ms = timed(lambda i: g.mc_update(0, q.ALGO_METROPOLIS, q.UPDATE_LINK, (i // 2) % 3, i % 2), nk)
+
+    def metro_sweep(i):
+        for mu in range(3):
+            for par in range(2):
+                g.mc_update(0, q.ALGO_METROPOLIS, q.UPDATE_LINK, mu, par)
+
+    me_ms = [t / 6 for t in bracketed(metro_sweep, ns)]
     for e in evs + [ev0, ev1]:
         lib.su2_event_destroy(e)
     if args.copy_bytes:
@@ -197,8 +198,7 @@ def gpu_bench(args):
     lib.su2_stream_destroy(stream)
     return dict(total_ms=total_ms, wall=wall, V=V, hb_ms=float(np.mean(hb_ms)), or_ms=float(np.mean(or_ms)),
                 hb_ms_min=float(np.min(hb_ms)), E=float(np.mean(meas[:, 0])), P=det_ok,
-                en_ms=float(np.mean(en_ms[1:] if len(en_ms) > 1 else en_ms)),
-                me_ms=float(np.mean(me_ms[6:] if len(me_ms) > 6 else me_ms)))
+                en_ms=float(np.mean(en_ms)), me_ms=float(np.mean(me_ms[1:] if len(me_ms) > 1 else me_ms)))
 
 
 def cpu_baseline(args, V_gpu):
@@ -233,10 +233,11 @@ def cpu_baseline(args, V_gpu):
     return out
 
 
-def lattice_launch_bench(Lt, Lx, Ly, reps=3):
+def lattice_launch_bench(Lt, Lx, Ly, reps=5):
     """Heat-bath and over-relaxation launches of one Lt x Lx x Ly lattice
-    (FP64 links, periodic, beta = 5), each timed with HIP events on the
-    launch stream after two warm-up sweeps."""
+    (FP64 links, periodic, beta = 5), after two warm-up sweeps: the six
+    launches of a sweep bracketed by one pair of HIP events on the launch
+    stream, per launch = bracket / 6."""
     import qcd_2d_gpu_amd as q
     from qcd_2d_gpu_amd import _lib as L
 
@@ -253,20 +254,20 @@ def lattice_launch_bench(Lt, Lx, Ly, reps=3):
     for upd, name, nbytes in ((q.UPDATE_LINK, "heatbath", BYTES_HB), (q.UPDATE_OVERRELAX, "overrelax", BYTES_OR)):
         evs = []
         for _ in range(reps):
+            a, b = C.c_void_p(), C.c_void_p()
+            L.check(lib.su2_event_create(C.byref(a)), "event")
+            L.check(lib.su2_event_create(C.byref(b)), "event")
+            L.check(lib.su2_event_record(a, stream), "record")
             for mu in range(3):
                 for par in range(2):
-                    a, b = C.c_void_p(), C.c_void_p()
-                    lib.su2_event_create(C.byref(a))
-                    lib.su2_event_create(C.byref(b))
-                    lib.su2_event_record(a, stream)
                     g.mc_update(0, q.ALGO_HEATBATH, upd, mu, par)
-                    lib.su2_event_record(b, stream)
-                    evs.append((a, b))
+            L.check(lib.su2_event_record(b, stream), "record")
+            evs.append((a, b))
         g.synchronize()
         ms, v = C.c_float(), []
         for a, b in evs:
             lib.su2_event_elapsed_ms(a, b, C.byref(ms))
-            v.append(ms.value * 1e-3)
+            v.append(ms.value * 1e-3 / 6)
             lib.su2_event_destroy(a)
             lib.su2_event_destroy(b)
         t = float(np.mean(v))
