@@ -397,7 +397,28 @@ def batched_bench(args, precision):
            "results_identical": same, "precision": "FP64" if precision == 0 else "FP32 links"}
     if precision == 0 and args.cpu_steps != 0:
         out["cpu_sample"] = ensemble_cpu_sample(args, jobs[:8])
+    # per-job plaquette series, for the FP32-link error against FP64 (not printed)
+    out["_plaquette"] = [series[s]["P"] for s in slots]
     return out
+
+
+def fp32_ensemble_error(p64, p32, nbins=8):
+    """Each job of the ensemble run with FP32 links against the same job with
+    FP64 links (same seed): the difference of the mean plaquettes in units
+    of the combined binned error (nbins bins of the measured sweeps)."""
+    from qcd_2d_gpu_amd import stats
+
+    z, d = [], []
+    for a, b in zip(p64, p32):
+        m64, e64 = stats.binned_error(a, nbins)
+        m32, e32 = stats.binned_error(b, nbins)
+        sig = float(np.hypot(e64, e32))
+        d.append(m32 - m64)
+        z.append((m32 - m64) / sig if sig > 0 else 0.0)
+    za = np.abs(np.array(z))
+    return {"jobs": len(z), "bins": nbins, "max_abs_diff": float(np.max(np.abs(d))),
+            "max_abs_diff_over_sigma": float(za.max()), "within_1_sigma": float(np.mean(za <= 1.0)),
+            "within_2_sigma": float(np.mean(za <= 2.0)), "mean_diff_over_sigma": float(np.mean(z))}
 
 
 def ensemble_cpu_sample(args, jobs):
@@ -558,7 +579,9 @@ def main():
               "the single-lattice sweep has no multi-GPU path here", file=sys.stderr)
         sys.exit(2)
     if args.batched_only:
-        print(json.dumps(batched_bench(args, args.precision)))
+        r = batched_bench(args, args.precision)
+        r.pop("_plaquette")
+        print(json.dumps(r))
         return
     if args.serving_only:
         out = {"serving": serving_bench(args)}
@@ -584,6 +607,10 @@ def main():
     if args.batch_jobs:
         extra["batched_ensemble_fp64"] = batched_bench(args, 0)
         extra["batched_ensemble_fp32"] = batched_bench(args, 1)
+        p64 = extra["batched_ensemble_fp64"].pop("_plaquette")
+        p32 = extra["batched_ensemble_fp32"].pop("_plaquette")
+        if args.batch_meas >= 8:
+            extra["batched_ensemble_fp32"]["plaquette_vs_fp64"] = fp32_ensemble_error(p64, p32)
     if args.serve_requests:
         extra["serving"] = serving_bench(args)
     if args.serve_requests and args.serve_burst:

@@ -81,3 +81,20 @@ def test_small_configs_are_the_reference_configurations():
         assert (kw["Lt"], kw["Lx"], kw["Ly"], kw["beta"], kw["algo"], kw["epsilon"], kw["upd"]) == \
             (32, 32, 48, 4.0, q.ALGO_METROPOLIS, 0.5, 4)
     assert {kw["boundary"] for _, _, kw in bench.SMALL_CONFIGS[1:]} == {q.BC_PCC, q.BC_PMC}
+
+
+def test_fp32_ensemble_error_record():
+    """The per-job FP32-link error: identical series give zero, a shift of
+    1.5 combined binned errors gives 1.5 sigma for that job."""
+    rng = np.random.default_rng(4)
+    p64 = [0.6 + 0.01 * rng.normal(size=80) for _ in range(5)]
+    same = bench.fp32_ensemble_error(p64, [a.copy() for a in p64])
+    assert same["jobs"] == 5 and same["max_abs_diff"] == 0.0 and same["within_1_sigma"] == 1.0
+    from qcd_2d_gpu_amd import stats
+
+    p32 = [a.copy() for a in p64]
+    m, e = stats.binned_error(p64[2], 8)
+    p32[2] = p64[2] + 1.5 * np.sqrt(2.0) * e  # the same error on both sides: combined sigma = sqrt(2) e
+    out = bench.fp32_ensemble_error(p64, p32)
+    assert abs(out["max_abs_diff_over_sigma"] - 1.5) < 1e-9
+    assert out["within_1_sigma"] == 0.8 and out["within_2_sigma"] == 1.0
