@@ -105,16 +105,17 @@ __device__ static inline bool normalize_post(const su2_bdesc &d, uint32_t s)
  * workgroup-shared rejection loop, so every thread of the workgroup calls
  * this for each virtual block (in_range = 0 past the end of the lattice);
  * the algorithm is uniform over a virtual block. */
-template <int MU, int BC, int ALGO, typename LinkT>
+template <int MU, int ALGO, typename LinkT>
 __device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t h, int in_range, su2_kp_queue &q)
 {
     if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
         /* early RNG load for FP32 links only (measured: batched FP32 -4.5 %,
          * FP64 +1.6 % per ensemble run; see su2_sweep_site_hb_wg) */
-        su2_sweep_site_hb_wg<MU, BC, LinkT, sizeof(LinkT) == sizeof(su2_l32)>((LinkT *)d.U, d.rng, d.accept, d.g,
-                                                                              d.ph, parity, h, in_range, q);
+        su2_sweep_site_hb_wg<MU, SU2_K_BC_ANY, LinkT, sizeof(LinkT) == sizeof(su2_l32)>(
+            (LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, in_range, q, d.bc);
     else if (in_range)
-        su2_sweep_site<MU, BC, SU2_K_METRO, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
+        su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_METRO, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h,
+                                                            d.bc);
 }
 
 /* OR = 0: link update; ALGO is the algorithm class of the resident lattices
@@ -137,17 +138,11 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
             if (or_pass >= d.or_sweeps) continue;
             const uint32_t h = (vb - d.sw_begin) * blockDim.x + threadIdx.x;
             if (h >= d.g.Vh) continue;
-            switch (d.bc) {
-            case SU2_K_BC_PER:
-                su2_sweep_site<MU, SU2_K_BC_PER, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
-                break;
-            case SU2_K_BC_PCC:
-                su2_sweep_site<MU, SU2_K_BC_PCC, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
-                break;
-            default:
-                su2_sweep_site<MU, SU2_K_BC_PMC, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h);
-                break;
-            }
+            /* one body for every boundary: the boundary is a run-time test
+             * (uniform in the workgroup), which keeps the register count of
+             * a single update instead of the largest of three inlined ones */
+            su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h,
+                                                             d.bc);
         }
     } else {
         __shared__ su2_kp_queue q;
@@ -156,11 +151,7 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
             const su2_bdesc &d = D[find_desc(B, s_begin, lds, n, vb)];
             const uint32_t h = (vb - d.sw_begin) * blockDim.x + threadIdx.x;
             const int in_range = h < d.g.Vh;
-            switch (d.bc) {
-            case SU2_K_BC_PER: site_link<MU, SU2_K_BC_PER, ALGO, LinkT>(d, parity, h, in_range, q); break;
-            case SU2_K_BC_PCC: site_link<MU, SU2_K_BC_PCC, ALGO, LinkT>(d, parity, h, in_range, q); break;
-            default: site_link<MU, SU2_K_BC_PMC, ALGO, LinkT>(d, parity, h, in_range, q); break;
-            }
+            site_link<MU, ALGO, LinkT>(d, parity, h, in_range, q);
         }
     }
 }

@@ -33,6 +33,10 @@
 
 enum { SU2_DIR_Y = 0, SU2_DIR_X = 1, SU2_DIR_T = 2 };
 enum { SU2_K_BC_PER = 0, SU2_K_BC_PCC = 1, SU2_K_BC_PMC = 2 };
+/* Template value for a boundary condition chosen at run time (the batched
+ * launches, whose virtual blocks belong to lattices of any boundary): the
+ * site code then tests the bc argument instead of a constant. */
+enum { SU2_K_BC_ANY = -1 };
 enum { SU2_K_METRO = 0, SU2_K_HB = 1, SU2_K_OR = 2 };
 
 struct __attribute__((aligned(32))) su2_l64 {

@@ -69,13 +69,15 @@ __device__ static inline __attribute__((always_inline)) uint32_t su2_hidx(const 
 
 /* The staple A of U_mu at half-site h of parity p; returns 0 (and leaves A
  * alone) for a link the boundary condition does not update (PCC: x and t
- * links at y = 0). */
+ * links at y = 0).  BC = SU2_K_BC_ANY takes the boundary from bc. */
 template <int MU, int BC, typename LinkT>
 __device__ static inline __attribute__((always_inline)) int su2_site_staple(const LinkT *__restrict__ U,
                                                                             const su2_kgeom &g, int parity,
-                                                                            uint32_t h, su2_q *Aout)
+                                                                            uint32_t h, su2_q *Aout, int bc = BC)
 {
     typedef su2_link_io<LinkT> io;
+    const bool pcc = BC == SU2_K_BC_PCC || (BC == SU2_K_BC_ANY && bc == SU2_K_BC_PCC);
+    const bool pmc = BC == SU2_K_BC_PMC || (BC == SU2_K_BC_ANY && bc == SU2_K_BC_PMC);
 
     const uint32_t q = su2_div(h, g.div_lyh);
     const uint32_t yh = h - q * g.Lyh;
@@ -83,7 +85,7 @@ __device__ static inline __attribute__((always_inline)) int su2_site_staple(cons
     const uint32_t x = q - t * g.Lx;
     const uint32_t y = 2u * yh + (((t + x) & 1u) ^ (uint32_t)parity);
 
-    if (BC == SU2_K_BC_PCC && MU != SU2_DIR_Y && y == 0u) return 0;
+    if (pcc && MU != SU2_DIR_Y && y == 0u) return 0;
 
     /* wrapped neighbour coordinates */
     const uint32_t yp = (y + 1u == g.Ly) ? 0u : y + 1u;
@@ -122,7 +124,7 @@ __device__ static inline __attribute__((always_inline)) int su2_site_staple(cons
 
         su2_q up = su2_q_mul_nd(su2_q_mul_nd(u_nu_pmu, u_mu_pnu), u_nu_n);
         su2_q dn = su2_q_mul(su2_q_mul_dd(u_nu_pmu_mnu, u_mu_mnu), u_nu_mnu);
-        if (BC == SU2_K_BC_PMC && MU != SU2_DIR_Y && NU == SU2_DIR_Y) {
+        if (pmc && MU != SU2_DIR_Y && NU == SU2_DIR_Y) {
             /* A is never -0 here, so adding +0 leaves it bit-identical. */
             if (y == g.Ly - 1u) up = su2_q_zero();
             if (y == 0u) dn = su2_q_zero();
@@ -139,12 +141,12 @@ __device__ static inline __attribute__((always_inline)) void su2_sweep_site(Link
                                                                             uint32_t *__restrict__ rng,
                                                                             int *__restrict__ accept,
                                                                             const su2_kgeom &g, const su2_kphys &ph,
-                                                                            int parity, uint32_t h)
+                                                                            int parity, uint32_t h, int bc = BC)
 {
     typedef su2_link_io<LinkT> io;
 
     su2_q A;
-    if (!su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A)) return;
+    if (!su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A, bc)) return;
     const size_t Vh = g.Vh;
     LinkT *dst = U + (size_t)(parity * 3 + MU) * g.Vh + h;
     int acc;
@@ -200,7 +202,7 @@ template <int MU, int BC, typename LinkT, bool EARLY_RNG = true>
 __device__ static inline __attribute__((always_inline)) void
 su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept,
                      const su2_kgeom &g, const su2_kphys &ph, int parity, uint32_t h, int in_range,
-                     su2_kp_queue &q)
+                     su2_kp_queue &q, int bc = BC)
 {
     typedef su2_link_io<LinkT> io;
 
@@ -223,7 +225,7 @@ su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__r
         }
     }
     su2_q A = su2_q_zero();
-    const int active = in_range ? su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A) : 0;
+    const int active = in_range ? su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A, bc) : 0;
     const size_t Vh = g.Vh;
     float inv = 0.0f, lf = 0.0f;
     int done = 1;
