@@ -36,7 +36,10 @@ namespace {
  * instead of two full planes (2.6 MB at 128^3) later.  Measured with FP64
  * links at 128^3: over-relaxation launch -10 %, heat-bath -5 %; FP32 links
  * (half the footprint) and planes of more than 64 blocks (256^3,
- * DRAM-bound) gain nothing from it and keep the slab order.
+ * DRAM-bound) gain nothing from it and keep the slab order.  Strips are
+ * 16 blocks wide where the plane's block count allows it, else 8: at 128^3
+ * (32 blocks per plane) 16 against 8 gave heat-bath -4 %, step -2 %; 4 and
+ * 32 were slower than 8.
  *
  * ORDER_COLUMNS (thin lattices, Lt <= 32, with bpp a multiple of 8): a slab
  * of Lt/8 <= 4 planes would send half or more of an XCD's t-neighbour reads
@@ -93,14 +96,15 @@ __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *
     sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp);
 }
 
-/* FP64-link sweeps run at most 4 waves per SIMD.  The heat-bath with the
- * shared rejection loop needs only 74-108 VGPRs and over-relaxation 66, so
- * the compiler would allow 5-7 waves, but more workgroups in flight spread
- * each XCD's working set over more t-planes.  Measured on one box, 4 waves
- * against uncapped: heat-bath 128^3 -4 %, 256^3 -9 % (one plane of the five
- * link arrays is already larger than an XCD's 4 MB L2 there);
- * over-relaxation -6 % and -5 %.  FP32 links (half the footprint) are as
- * fast or faster uncapped. */
+/* FP64-link sweeps run at most 4 waves per SIMD.  More workgroups in flight
+ * spread each XCD's working set over more t-planes.  Measured on one box
+ * when the kernels needed few enough VGPRs for 6-7 waves, 4 waves against
+ * uncapped: heat-bath 128^3 -4 %, 256^3 -9 % (one plane of the five link
+ * arrays is already larger than an XCD's 4 MB L2 there); over-relaxation
+ * -6 % and -5 %.  With the fused arithmetic (heat-bath 106 VGPRs,
+ * over-relaxation 94) a cap of 5 (100 VGPRs, no spills) timed the same as
+ * 4 at 128^3.  FP32 links (half the footprint) are as fast or faster
+ * uncapped. */
 template <int MU, int BC, int UPD, typename LinkT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void
 k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
@@ -117,7 +121,7 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
     /* block order for FP64 links when the lattice tiles evenly (see
      * sweep_block); FP32 links keep the slab order */
     uint32_t order = ORDER_SLAB, strip = 0, bpp = g.plane / (uint32_t)block;
-    const uint32_t w = 8;
+    const uint32_t w = (bpp % 16u == 0) ? 16u : 8u;
     const bool f64 = sizeof(LinkT) == sizeof(su2_l64);
     const bool whole = g.plane % (uint32_t)block == 0 && nb * (uint32_t)block == g.Vh;
     if (f64 && whole && g.Lt <= 32u && bpp % 8u == 0) {
