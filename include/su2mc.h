@@ -310,8 +310,8 @@ typedef struct su2_engine_config {
     int max_or;          /* over-relaxation passes per tick (0: 1) */
     size_t budget_bytes; /* device memory for job data (0: 1 GiB; at most 90 % of the free memory) */
     int use_graph;       /* replay ticks from a captured hipGraph */
-    int block;           /* sweep workgroup size: 64, 128 or 256 (0: autotuned) */
-    int grid;            /* persistent workgroups per launch (0: autotuned) */
+    int block;           /* sweep workgroup size: 64, 128, 192 or 256 (0: 256; su2_engine_autotune may change it) */
+    int grid;            /* most workgroups of a sweep launch (0: 16 per CU; su2_engine_autotune may change it) */
 } su2_engine_config;
 
 typedef struct su2_engine su2_engine;
