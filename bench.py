@@ -3,7 +3,7 @@
 
 Workload (the large single-lattice configuration): Lt = Lx = Ly = 128,
 periodic, heat-bath followed by one over-relaxation sweep, beta 5.0, hot
-start from seed 7, FP64 links.  One step = one heat-bath sweep (6 launches)
+start with Haar-random links from seed 7, FP64 links.  One step = one heat-bath sweep (6 launches)
 + one over-relaxation sweep (6 launches) + the fused energy/plaquette
 measurement, with the reunitarisation every 100 steps, as in the
 reference's measurement loop (kernel.cu:192-208).  Warm-up steps play the
@@ -88,7 +88,7 @@ def gpu_bench(args):
     g = q.Lattice(Lt, Lx, Ly, 5.0, 0.1, 1, precision=args.precision, stream=stream)
     V = g.V
     g.setup_kernel(7)
-    g.rand_init2()
+    g.haar_init()
     res = q.DeviceBuffer(8 * 2 * max(1, args.steps))
     res.zero(stream)
 
@@ -209,7 +209,7 @@ def cpu_baseline(args, V_gpu):
     L = args.L
     o = O.Oracle(L, L, L, 5.0)
     o.setup_kernel(7)
-    o.init(0)
+    o.start(2, 0)  # Haar-random hot start, as the GPU leg
     # one warm-up step, then time a bounded number of steps
     o.time_sweeps(0, 1, 1, 1)
     steps = args.cpu_steps
@@ -246,7 +246,7 @@ def lattice_launch_bench(Lt, Lx, Ly, reps=5):
     L.check(lib.su2_stream_create(C.byref(stream)), "stream_create")
     g = q.Lattice(Lt, Lx, Ly, 5.0, precision=q.PREC_F64, stream=stream)
     g.setup_kernel(7)
-    g.rand_init2()
+    g.haar_init()
     for _ in range(2):
         g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
         g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
@@ -633,10 +633,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": prec,
-        "data": "synthetic (hot start, MRG32k3a seed 7)",
+        "data": "synthetic (Haar-random hot start, MRG32k3a seed 7)",
         "config": {"workload": f"SU(2) 2+1D Wilson action, Lt=Lx=Ly={args.L}, periodic, beta=5.0, "
                                "heat-bath + 1 over-relaxation sweep per step + energy/plaquette measurement",
                    "Lt": args.L, "Lx": args.L, "Ly": args.L, "beta": 5.0, "boundary": "periodic",
+                   "start": "Haar-random SU(2) links, seed 7",
                    "links": "FP64" if args.precision == 0 else "FP32", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": hb_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hb_gbs / HBM_PEAK_GBS, "traffic": load_traffic(),
