@@ -2,8 +2,9 @@
 
 Runs hipcc directly, in-tree: objects go to qcd_2d_gpu_amd/_build/, the
 library and the executable to qcd_2d_gpu_amd/.  Every translation unit is
-compiled with floating-point contraction off (the bit-exact contract of the
-update bodies) and with packed FP32 instructions disabled.
+compiled with automatic floating-point contraction off, so the only fused
+multiply-adds are the ones the update bodies write out (the bit-exact
+contract with the CPU checker), and with packed FP32 instructions disabled.
 """
 from __future__ import annotations
 
