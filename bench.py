@@ -3,17 +3,17 @@
 
 Workload (the large single-lattice configuration): Lt = Lx = Ly = 128,
 periodic, heat-bath followed by one over-relaxation sweep, beta 5.0, hot
-start with Haar-random links from seed 7, FP64 links.  One step = one heat-bath sweep (6 launches)
-+ one over-relaxation sweep (6 launches) + the fused energy/plaquette
-measurement, with the reunitarisation every 100 steps, as in the
-reference's measurement loop (kernel.cu:192-208).  Warm-up steps play the
-role of thermalisation.
+start with Haar-random links from seed 7, FP64 links.  One step = one
+heat-bath sweep (6 launches) + one over-relaxation sweep (6 launches) + the
+fused energy/plaquette measurement, with the reunitarisation every 100
+steps, as in the reference's measurement loop (kernel.cu:192-208).  Warm-up
+steps play the role of thermalisation.
 
 Prints ONE JSON line.  value = link updates per second of the whole step
 (2 x 3 V updates per step).  roofline: the heat-bath sweep kernel, 240
 algorithmic bytes per link update (5 link reads + 1 link write of 32 B and
-48 B of RNG state read + written), timed per launch with HIP events on the
-stream it runs on.  cpu_baseline: the CPU checker (OpenMP C restatement of
+48 B of RNG state read + written), timed with HIP events around the six
+launches of a sweep on the stream they run on.  cpu_baseline: the CPU checker (OpenMP C restatement of
 the reference) on the host cores, on a bounded number of steps of the same
 lattice.
 """
