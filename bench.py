@@ -149,21 +149,17 @@ def gpu_bench(args):
             out.append(float(ms.value))
         return out
 
+    # whole sweeps through the sweep entry point, as in the step (it counts
+    # acceptance once per sweep; single launches count in each)
     def sweep_pair(i):
-        for mu in range(3):
-            for par in range(2):
-                g.mc_update(0, q.ALGO_HEATBATH, q.UPDATE_LINK, mu, par)
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
         L.check(lib.su2_event_record(evs[2 * i + 1], stream), "record")
-        for mu in range(3):
-            for par in range(2):
-                g.mc_update(0, q.ALGO_HEATBATH, q.UPDATE_OVERRELAX, mu, par)
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
 
     hb_ms = [t / 6 for t in bracketed(sweep_pair, ns, ends_itself=True)]
 
     def or_sweep(i):
-        for mu in range(3):
-            for par in range(2):
-                g.mc_update(0, q.ALGO_HEATBATH, q.UPDATE_OVERRELAX, mu, par)
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
 
     or_ms = [t / 6 for t in bracketed(or_sweep, ns)]
 
@@ -177,9 +173,7 @@ def gpu_bench(args):
     g.set_params(epsilon=0.5, updates=4)
 
     def metro_sweep(i):
-        for mu in range(3):
-            for par in range(2):
-                g.mc_update(0, q.ALGO_METROPOLIS, q.UPDATE_LINK, mu, par)
+        g.mc_up_per(q.ALGO_METROPOLIS, q.UPDATE_LINK)
 
     me_ms = [t / 6 for t in bracketed(metro_sweep, ns)]
     for e in evs + [ev0, ev1]:
@@ -258,9 +252,7 @@ def lattice_launch_bench(Lt, Lx, Ly, reps=5):
             L.check(lib.su2_event_create(C.byref(a)), "event")
             L.check(lib.su2_event_create(C.byref(b)), "event")
             L.check(lib.su2_event_record(a, stream), "record")
-            for mu in range(3):
-                for par in range(2):
-                    g.mc_update(0, q.ALGO_HEATBATH, upd, mu, par)
+            g.mc_up_per(q.ALGO_HEATBATH, upd)
             L.check(lib.su2_event_record(b, stream), "record")
             evs.append((a, b))
         g.synchronize()

@@ -103,10 +103,16 @@ int sweep(const su2_lattice *l, void *phi, int prec, int *accept, uint32_t *stat
     p.eps = (double)l->epsilon;
     p.hits = l->updates;
     const int upd = update == SU2_UPDATE_OVERRELAX ? SU2_K_OR : (algo == SU2_ALGO_HEATBATH ? SU2_K_HB : SU2_K_METRO);
-    /* (y,e) (y,o) (x,e) (x,o) (t,e) (t,o): functions/mc_updates.h:467-474 */
+    /* (y,e) (y,o) (x,e) (x,o) (t,e) (t,o): functions/mc_updates.h:467-474.
+     * Heat-bath and over-relaxation accept every update they make, so the
+     * t launches add each half-site's count for the whole sweep and the y
+     * and x launches count nothing; Metropolis counts in every launch. */
     for (int mu = 0; mu < 3; ++mu)
         for (int par = 0; par < 2; ++par) {
-            hipError_t e = su2_launch_sweep(mu, bc, upd, prec, phi, states, accept, g, p, par, g_sweep_block, S(stream));
+            const int acc_mode = upd == SU2_K_METRO ? SU2_K_ACC_LAUNCH
+                                                    : (mu == SU2_DIR_T ? SU2_K_ACC_SWEEP : SU2_K_ACC_NONE);
+            hipError_t e =
+                su2_launch_sweep(mu, bc, upd, prec, phi, states, accept, g, p, par, g_sweep_block, acc_mode, S(stream));
             if (e != hipSuccess) return hip_fail(e, "sweep launch");
         }
     return SU2_OK;
@@ -267,7 +273,8 @@ int su2_mc_update(const su2_lattice *l, void *phi, int precision, int *accept, u
     p.hits = l->updates;
     const int upd = update == SU2_UPDATE_OVERRELAX ? SU2_K_OR : (algo == SU2_ALGO_HEATBATH ? SU2_K_HB : SU2_K_METRO);
     hipError_t e =
-        su2_launch_sweep(mu, bc, upd, precision, phi, states, accept, geom(l), p, parity, g_sweep_block, S(stream));
+        su2_launch_sweep(mu, bc, upd, precision, phi, states, accept, geom(l), p, parity, g_sweep_block,
+                         SU2_K_ACC_LAUNCH, S(stream));
     return e == hipSuccess ? SU2_OK : hip_fail(e, "sweep launch");
 }
 

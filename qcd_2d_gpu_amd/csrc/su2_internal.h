@@ -38,6 +38,18 @@ enum { SU2_K_BC_PER = 0, SU2_K_BC_PCC = 1, SU2_K_BC_PMC = 2 };
  * site code then tests the bc argument instead of a constant. */
 enum { SU2_K_BC_ANY = -1 };
 enum { SU2_K_METRO = 0, SU2_K_HB = 1, SU2_K_OR = 2 };
+/* Acceptance counting of a sweep launch (argument acc_mode):
+ *   SU2_K_ACC_LAUNCH  each updated link adds its accepted count (the
+ *                     reference's per-kernel accept[site] += ...);
+ *   SU2_K_ACC_NONE    no counting;
+ *   SU2_K_ACC_SWEEP   each half-site adds the whole sweep's count of its
+ *                     parity: 3 updated links, 1 at PCC's y = 0 (only its y
+ *                     link is updated).  For heat-bath and over-relaxation,
+ *                     which accept every update, the six launches of a sweep
+ *                     run (y, x) with NONE and t with SWEEP: after the sweep
+ *                     every counter holds what per-launch counting gives, with
+ *                     one read-modify-write per site instead of three. */
+enum { SU2_K_ACC_LAUNCH = 0, SU2_K_ACC_NONE = 1, SU2_K_ACC_SWEEP = 2 };
 
 struct __attribute__((aligned(32))) su2_l64 {
     double a0, a1, a2, a3;
@@ -125,7 +137,8 @@ static inline uint32_t su2_grid_blocks(uint64_t nthreads, uint32_t block)
 
 /* ---- launchers (host), defined in the .hip files ---- */
 hipError_t su2_launch_sweep(int mu, int bc, int upd, int f32, void *U, uint32_t *rng, int *accept,
-                            const su2_kgeom &g, const su2_kphys &p, int parity, int block, hipStream_t st);
+                            const su2_kgeom &g, const su2_kphys &p, int parity, int block, int acc_mode,
+                            hipStream_t st);
 hipError_t su2_launch_setup_rng(uint32_t *rng, const su2_kgeom &g, uint64_t seed, hipStream_t st);
 /* start: 0 reference hot start, 1 cold, 2 reference PCC hot start, 3 Haar,
  * 4 Haar with PCC identity links at y = 0 */

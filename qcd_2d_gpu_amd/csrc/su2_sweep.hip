@@ -74,26 +74,26 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
                                                                         int *__restrict__ accept, const su2_kgeom &g,
                                                                         const su2_kphys &ph, int parity,
                                                                         uint32_t order, uint32_t strip,
-                                                                        uint32_t bpp)
+                                                                        uint32_t bpp, int acc_mode)
 {
     const uint32_t h = sweep_block(order, strip, bpp) * blockDim.x + threadIdx.x;
     if constexpr (UPD == SU2_K_HB) {
         __shared__ su2_kp_queue q;
         /* FP64 links: early RNG load (see su2_sweep_site_hb_wg) */
         su2_sweep_site_hb_wg<MU, BC, LinkT, sizeof(LinkT) == sizeof(su2_l64)>(U, rng, accept, g, ph, parity, h,
-                                                                              h < g.Vh, q);
+                                                                              h < g.Vh, q, BC, acc_mode);
     } else {
         if (h >= g.Vh) return;
-        su2_sweep_site<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, h);
+        su2_sweep_site<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, h, BC, acc_mode);
     }
 }
 
 template <int MU, int BC, int UPD, typename LinkT>
 __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *__restrict__ rng,
                                                int *__restrict__ accept, su2_kgeom g, su2_kphys ph, int parity,
-                                               uint32_t order, uint32_t strip, uint32_t bpp)
+                                               uint32_t order, uint32_t strip, uint32_t bpp, int acc_mode)
 {
-    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp);
+    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp, acc_mode);
 }
 
 /* FP64-link sweeps run at most 4 waves per SIMD.  More workgroups in flight
@@ -108,14 +108,14 @@ __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *
 template <int MU, int BC, int UPD, typename LinkT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void
 k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
-           int parity, uint32_t order, uint32_t strip, uint32_t bpp)
+           int parity, uint32_t order, uint32_t strip, uint32_t bpp, int acc_mode)
 {
-    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp);
+    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp, acc_mode);
 }
 
 template <int MU, int BC, int UPD, typename LinkT>
 hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, const su2_kphys &p, int parity,
-                      int block, hipStream_t st)
+                      int block, int acc_mode, hipStream_t st)
 {
     const uint32_t nb = su2_grid_blocks(g.Vh, (uint32_t)block);
     /* block order for FP64 links when the lattice tiles evenly (see
@@ -132,45 +132,48 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
     }
     if constexpr (sizeof(LinkT) == sizeof(su2_l64))
         hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
-                           p, parity, order, strip, bpp);
+                           p, parity, order, strip, bpp, acc_mode);
     else
         hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
-                           p, parity, order, strip, bpp);
+                           p, parity, order, strip, bpp, acc_mode);
     return hipGetLastError();
 }
 
 template <int BC, int UPD, typename LinkT>
 hipError_t launch_mu(int mu, void *U, uint32_t *rng, int *accept, const su2_kgeom &g, const su2_kphys &p, int parity,
-                     int block, hipStream_t st)
+                     int block, int acc_mode, hipStream_t st)
 {
     switch (mu) {
-    case SU2_DIR_Y: return launch_one<SU2_DIR_Y, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, st);
-    case SU2_DIR_X: return launch_one<SU2_DIR_X, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, st);
-    case SU2_DIR_T: return launch_one<SU2_DIR_T, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, st);
+    case SU2_DIR_Y: return launch_one<SU2_DIR_Y, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, st);
+    case SU2_DIR_X: return launch_one<SU2_DIR_X, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, st);
+    case SU2_DIR_T: return launch_one<SU2_DIR_T, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, st);
     }
     return hipErrorInvalidValue;
 }
 
 template <int BC, typename LinkT>
 hipError_t launch_upd(int mu, int upd, void *U, uint32_t *rng, int *accept, const su2_kgeom &g, const su2_kphys &p,
-                      int parity, int block, hipStream_t st)
+                      int parity, int block, int acc_mode, hipStream_t st)
 {
     switch (upd) {
-    case SU2_K_METRO: return launch_mu<BC, SU2_K_METRO, LinkT>(mu, U, rng, accept, g, p, parity, block, st);
-    case SU2_K_HB: return launch_mu<BC, SU2_K_HB, LinkT>(mu, U, rng, accept, g, p, parity, block, st);
-    case SU2_K_OR: return launch_mu<BC, SU2_K_OR, LinkT>(mu, U, rng, accept, g, p, parity, block, st);
+    case SU2_K_METRO: return launch_mu<BC, SU2_K_METRO, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, st);
+    case SU2_K_HB: return launch_mu<BC, SU2_K_HB, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, st);
+    case SU2_K_OR: return launch_mu<BC, SU2_K_OR, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, st);
     }
     return hipErrorInvalidValue;
 }
 
 template <typename LinkT>
 hipError_t launch_bc(int mu, int bc, int upd, void *U, uint32_t *rng, int *accept, const su2_kgeom &g,
-                     const su2_kphys &p, int parity, int block, hipStream_t st)
+                     const su2_kphys &p, int parity, int block, int acc_mode, hipStream_t st)
 {
     switch (bc) {
-    case SU2_K_BC_PER: return launch_upd<SU2_K_BC_PER, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, st);
-    case SU2_K_BC_PCC: return launch_upd<SU2_K_BC_PCC, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, st);
-    case SU2_K_BC_PMC: return launch_upd<SU2_K_BC_PMC, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, st);
+    case SU2_K_BC_PER:
+        return launch_upd<SU2_K_BC_PER, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, st);
+    case SU2_K_BC_PCC:
+        return launch_upd<SU2_K_BC_PCC, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, st);
+    case SU2_K_BC_PMC:
+        return launch_upd<SU2_K_BC_PMC, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, st);
     }
     return hipErrorInvalidValue;
 }
@@ -178,8 +181,9 @@ hipError_t launch_bc(int mu, int bc, int upd, void *U, uint32_t *rng, int *accep
 } // namespace
 
 hipError_t su2_launch_sweep(int mu, int bc, int upd, int f32, void *U, uint32_t *rng, int *accept,
-                            const su2_kgeom &g, const su2_kphys &p, int parity, int block, hipStream_t st)
+                            const su2_kgeom &g, const su2_kphys &p, int parity, int block, int acc_mode,
+                            hipStream_t st)
 {
-    if (f32) return launch_bc<su2_l32>(mu, bc, upd, U, rng, accept, g, p, parity, block, st);
-    return launch_bc<su2_l64>(mu, bc, upd, U, rng, accept, g, p, parity, block, st);
+    if (f32) return launch_bc<su2_l32>(mu, bc, upd, U, rng, accept, g, p, parity, block, acc_mode, st);
+    return launch_bc<su2_l64>(mu, bc, upd, U, rng, accept, g, p, parity, block, acc_mode, st);
 }

@@ -141,12 +141,18 @@ __device__ static inline __attribute__((always_inline)) void su2_sweep_site(Link
                                                                             uint32_t *__restrict__ rng,
                                                                             int *__restrict__ accept,
                                                                             const su2_kgeom &g, const su2_kphys &ph,
-                                                                            int parity, uint32_t h, int bc = BC)
+                                                                            int parity, uint32_t h, int bc = BC,
+                                                                            int acc_mode = SU2_K_ACC_LAUNCH)
 {
     typedef su2_link_io<LinkT> io;
 
     su2_q A;
-    if (!su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A, bc)) return;
+    if (!su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A, bc)) {
+        /* a link the boundary does not update (PCC x/t at y = 0): its
+         * half-site's sweep count is the y link's update alone */
+        if (acc_mode == SU2_K_ACC_SWEEP) atomicAdd(&accept[(size_t)parity * g.Vh + h], 1);
+        return;
+    }
     const size_t Vh = g.Vh;
     LinkT *dst = U + (size_t)(parity * 3 + MU) * g.Vh + h;
     int acc;
@@ -173,7 +179,10 @@ __device__ static inline __attribute__((always_inline)) void su2_sweep_site(Link
      * counter before it ends (each counter belongs to one thread per
      * launch, so the sum is the same; heat-bath -1.6 %, over-relaxation
      * -1 % to -3 % per launch). */
-    atomicAdd(&accept[(size_t)parity * Vh + h], acc);
+    if (acc_mode == SU2_K_ACC_LAUNCH)
+        atomicAdd(&accept[(size_t)parity * Vh + h], acc);
+    else if (acc_mode == SU2_K_ACC_SWEEP)
+        atomicAdd(&accept[(size_t)parity * Vh + h], 3);
 }
 
 /*
@@ -202,7 +211,7 @@ template <int MU, int BC, typename LinkT, bool EARLY_RNG = true>
 __device__ static inline __attribute__((always_inline)) void
 su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept,
                      const su2_kgeom &g, const su2_kphys &ph, int parity, uint32_t h, int in_range,
-                     su2_kp_queue &q, int bc = BC)
+                     su2_kp_queue &q, int bc = BC, int acc_mode = SU2_K_ACC_LAUNCH)
 {
     typedef su2_link_io<LinkT> io;
 
@@ -271,7 +280,13 @@ su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__r
         io::store(dst, su2_kp_finish(A, lf, s));
 #pragma unroll
         for (int i = 0; i < 6; ++i) rng[(size_t)i * Vh + h] = s[i];
-        atomicAdd(&accept[(size_t)parity * Vh + h], 1); /* no-return add, see su2_sweep_site */
+        /* no-return adds, see su2_sweep_site */
+        if (acc_mode == SU2_K_ACC_LAUNCH)
+            atomicAdd(&accept[(size_t)parity * Vh + h], 1);
+        else if (acc_mode == SU2_K_ACC_SWEEP)
+            atomicAdd(&accept[(size_t)parity * Vh + h], 3);
+    } else if (in_range && acc_mode == SU2_K_ACC_SWEEP) {
+        atomicAdd(&accept[(size_t)parity * Vh + h], 1); /* PCC x/t at y = 0: the y link's update */
     }
 }
 
