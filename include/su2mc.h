@@ -126,7 +126,10 @@ int su2_haar_init(const su2_lattice *l, void *phi, int precision, uint32_t *stat
  * the order (y,even) (y,odd) (x,even) (x,odd) (t,even) (t,odd).
  * algo = SU2_ALGO_HEATBATH or SU2_ALGO_METROPOLIS (the reference's compile-
  * time ALGO, kernel.cu:13); update = SU2_UPDATE_LINK or SU2_UPDATE_OVERRELAX
- * (the reference's `int update`).  accept[site] += accepted hits.
+ * (the reference's `int update`).  accept[site] += accepted hits, as the
+ * six reference kernels leave it after the sweep (the heat-bath and
+ * over-relaxation, which accept every update, add a site's count for the
+ * whole sweep in its t launch; in between, the counters are not defined).
  * Couplings (from *l): the heat-bath needs a finite beta > 0 (its
  * Kennedy-Pendleton loop accepts with probability of order beta |A| for
  * small beta, so runs below beta ~ 0.1 get slow, and never at beta <= 0);
@@ -148,7 +151,8 @@ int su2_mc_up_pmc(const su2_lattice *l, void *phi, int precision, int *accept, u
 
 /* One (direction, parity) launch of the above: mu 0 = y, 1 = x, 2 = t;
  * parity 0 = even, 1 = odd (one MC_update_{even,odd}_{y,x,t}[_pcc|_pmc]
- * kernel, functions/mc_updates.h:129-465). */
+ * kernel, functions/mc_updates.h:129-465); counts its own accepted updates
+ * into accept. */
 int su2_mc_update(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int bc, int algo,
                   int update, int mu, int parity, void *stream);
 
