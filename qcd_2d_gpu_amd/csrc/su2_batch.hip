@@ -105,6 +105,15 @@ __device__ static inline bool normalize_post(const su2_bdesc &d, uint32_t s)
  * workgroup-shared rejection loop, so every thread of the workgroup calls
  * this for each virtual block (in_range = 0 past the end of the lattice);
  * the algorithm is uniform over a virtual block. */
+/* Acceptance of heat-bath and over-relaxation, which accept every update,
+ * is counted once per sweep, in the t launches (SU2_K_ACC_SWEEP); Metropolis
+ * counts in every launch. */
+template <int MU>
+__device__ static inline constexpr int accept_once_per_sweep()
+{
+    return MU == SU2_DIR_T ? SU2_K_ACC_SWEEP : SU2_K_ACC_NONE;
+}
+
 template <int MU, int ALGO, typename LinkT>
 __device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t h, int in_range, su2_kp_queue &q)
 {
@@ -112,7 +121,7 @@ __device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t
         /* early RNG load for FP32 links only (measured: batched FP32 -4.5 %,
          * FP64 +1.6 % per ensemble run; see su2_sweep_site_hb_wg) */
         su2_sweep_site_hb_wg<MU, SU2_K_BC_ANY, LinkT, sizeof(LinkT) == sizeof(su2_l32)>(
-            (LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, in_range, q, d.bc);
+            (LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, in_range, q, d.bc, accept_once_per_sweep<MU>());
     else if (in_range)
         su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_METRO, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h,
                                                             d.bc);
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
              * (uniform in the workgroup), which keeps the register count of
              * a single update instead of the largest of three inlined ones */
             su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_OR, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h,
-                                                             d.bc);
+                                                             d.bc, accept_once_per_sweep<MU>());
         }
     } else {
         __shared__ su2_kp_queue q;
