@@ -590,31 +590,43 @@ def main():
     or_gbs = BYTES_OR * per_launch_updates / (r["or_ms"] * 1e-3) / 1e9
     step_bytes = 3 * V * (BYTES_HB + BYTES_OR) + 96.0 * V
     extra = {}
+
+    def section(name, fn):
+        """One optional part of the line; a failure there is recorded under
+        its key instead of losing the headline measurement."""
+        try:
+            extra[name] = fn()
+        except Exception as e:  # noqa: BLE001 - reported in the JSON line
+            extra[name] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"bench.py: {name} failed: {e}", file=sys.stderr)
+
     if args.large_L:
-        extra["large_lattice_fp64"] = large_lattice_bench(args)
+        section("large_lattice_fp64", lambda: large_lattice_bench(args))
     if args.large_L and args.shard_shares:
-        extra["shard_share_fp64"] = shard_share_bench()
+        section("shard_share_fp64", shard_share_bench)
     if args.small_configs and args.cpu_steps != 0:
-        extra["small_configs"] = small_config_bench(args)
+        section("small_configs", lambda: small_config_bench(args))
     if args.batch_jobs:
-        extra["batched_ensemble_fp64"] = batched_bench(args, 0)
-        extra["batched_ensemble_fp32"] = batched_bench(args, 1)
-        p64 = extra["batched_ensemble_fp64"].pop("_plaquette")
-        p32 = extra["batched_ensemble_fp32"].pop("_plaquette")
-        if args.batch_meas >= 8:
+        section("batched_ensemble_fp64", lambda: batched_bench(args, 0))
+        section("batched_ensemble_fp32", lambda: batched_bench(args, 1))
+        p64 = extra["batched_ensemble_fp64"].pop("_plaquette", None)
+        p32 = extra["batched_ensemble_fp32"].pop("_plaquette", None)
+        if args.batch_meas >= 8 and p64 is not None and p32 is not None:
             extra["batched_ensemble_fp32"]["plaquette_vs_fp64"] = fp32_ensemble_error(p64, p32)
     if args.serve_requests:
-        extra["serving"] = serving_bench(args)
+        section("serving", lambda: serving_bench(args))
     if args.serve_requests and args.serve_burst:
-        extra["serving_burst"] = serving_bench(args, burst=True)
+        section("serving_burst", lambda: serving_bench(args, burst=True))
     if args.fp32_meas:
-        extra["fp32_links_vs_fp64"] = fp32_check(args)
+        section("fp32_links_vs_fp64", lambda: fp32_check(args))
     cpu = None
     if args.cpu_steps != 0:
         cpu = cpu_baseline(args, V)
     prec = "f64" if args.precision == 0 else "f64 (FP32 link storage)"
     out = {
-        "metric": "link updates/s (SU(2) 128^3 FP64 heat-bath + over-relaxation sweep)",
+        "metric": "link updates/s + HBM GB/s (FP64 128\u00b3); \u03b2-scan jobs/s with p50/p99 latency",
+        "value_is": f"link updates/s of the {args.L}^3 step (heat-bath + over-relaxation sweep + en_av); "
+                    "HBM GB/s in roofline, jobs/s and p50/p99 in batched_ensemble_* and serving*",
         "value": ups,
         "unit": "link updates/s",
         "n_gpus": 1,
