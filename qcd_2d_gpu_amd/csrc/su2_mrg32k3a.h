@@ -66,17 +66,34 @@ SU2_MRG_FN uint64_t su2_mrg_fold2(uint64_t p)
     return p;
 }
 
+/* The last fold and conditional subtraction of the recurrence step, in
+ * 32-bit arithmetic: v = c * hi + lo reduced once by m = 2^32 - c.  With
+ * t = lo + c * (hi + 1) mod 2^32, v >= m exactly when that addition
+ * carries, and then v - m = t; otherwise v = t - c.  Equal to
+ * su2_mrg_fold1's (c = 209) and su2_mrg_fold2's last (c = 22853) fold and
+ * subtraction for every input the step produces.  hi_mask covers every
+ * value hi takes there (below 2^22 and 2^5); it only tells the compiler
+ * that c * (hi + 1) fits a 24-bit multiply with a 32-bit result. */
+SU2_MRG_FN uint32_t su2_mrg_tail(uint32_t hi, uint32_t lo, uint32_t c, uint32_t hi_mask)
+{
+    const uint32_t t = lo + ((hi & hi_mask) * c + c);
+    return (t < lo) ? t : t - c;
+}
+
 /* One step of both recurrences; s = {g1[0], g1[1], g1[2], g2[0], g2[1],
- * g2[2]}.  Returns the combined output in [1, m1]. */
+ * g2[2]}.  Returns the combined output in [1, m1].  The products are below
+ * 2^54, so the high words entering su2_mrg_tail are below 2^22 (component
+ * 1) and, after component 2's first 64-bit fold, below 2^5. */
 SU2_MRG_FN uint32_t su2_mrg_next(uint32_t *s)
 {
-    uint32_t p1 = (uint32_t)su2_mrg_fold1((uint64_t)SU2_MRG_A12 * s[1] +
-                                          (uint64_t)SU2_MRG_A13N * (uint32_t)(SU2_MRG_M1 - s[0]));
+    const uint64_t a = (uint64_t)SU2_MRG_A12 * s[1] + (uint64_t)SU2_MRG_A13N * (uint32_t)(SU2_MRG_M1 - s[0]);
+    const uint32_t p1 = su2_mrg_tail((uint32_t)(a >> 32), (uint32_t)a, SU2_MRG_M1C, 0x3fffffu);
     s[0] = s[1];
     s[1] = s[2];
     s[2] = p1;
-    uint32_t p2 = (uint32_t)su2_mrg_fold2((uint64_t)SU2_MRG_A21 * s[5] +
-                                          (uint64_t)SU2_MRG_A23N * (uint32_t)(SU2_MRG_M2 - s[3]));
+    uint64_t b = (uint64_t)SU2_MRG_A21 * s[5] + (uint64_t)SU2_MRG_A23N * (uint32_t)(SU2_MRG_M2 - s[3]);
+    b = (uint64_t)SU2_MRG_M2C * (uint32_t)(b >> 32) + (uint32_t)b;
+    const uint32_t p2 = su2_mrg_tail((uint32_t)(b >> 32), (uint32_t)b, SU2_MRG_M2C, 0x1fu);
     s[3] = s[4];
     s[4] = s[5];
     s[5] = p2;

@@ -115,3 +115,22 @@ def test_uniform_range_and_mean():
     u = np.array([lib.shim_uniform_double(s.ctypes.data) for _ in range(20000)])
     assert u.min() > 0.0 and u.max() <= 1.0
     assert abs(u.mean() - 0.5) < 5 * np.sqrt(1 / 12 / u.size)
+
+
+def test_step_equals_checker_on_edge_states(oracle_mod):
+    """The product's recurrence step (32-bit carry reduction) against the
+    checker's (64-bit folds) from states at and around the moduli, zero, and
+    beyond the moduli (words the step must reduce the same way), and from
+    random states: every draw and every state word equal."""
+    lib = _shim.load()
+    o = oracle_mod.load()
+    rng = np.random.default_rng(5)
+    edge = [0, 1, 2, 208, 209, 22852, 22853, M2 - 1, M2, M2 + 1, M1 - 1, M1, M1 + 1, 2 ** 32 - 1, 2 ** 31]
+    states = [rng.choice(edge, 6) for _ in range(400)] + [rng.integers(0, 2 ** 32, 6) for _ in range(400)]
+    for st in states:
+        a = np.array(st, dtype=np.uint32)
+        b = a.copy()
+        for _ in range(4):
+            x = lib.shim_mrg_next(a.ctypes.data)
+            y = o.orc_mrg_next(b.ctypes.data)
+            assert x == y and np.array_equal(a, b), (list(st), x, y)
