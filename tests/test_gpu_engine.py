@@ -187,6 +187,11 @@ def test_server_continuous_batching(gpu):
             assert np.array_equal(series[i][1].view(np.uint64), P.view(np.uint64)), f"job {i} plaquettes"
             assert abs(r.E_mean - E.mean()) < 1e-12 and abs(r.P_mean - P.mean()) < 1e-12
             assert abs(r.accept - acc) < 1e-12
+        # the mix really happened: some jobs were resident together, and some
+        # waited for memory (three fit in the budget, twelve were submitted)
+        iv = sorted((res[i].t_admit, res[i].t_done) for i in ids)
+        assert any(nxt[0] < cur[1] for cur, nxt in zip(iv, iv[1:])), iv
+        assert max(res[i].t_admit - res[i].t_submit for i in ids) > 0.0
     finally:
         srv.close()
 
