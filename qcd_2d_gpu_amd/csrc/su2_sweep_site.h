@@ -219,13 +219,12 @@ su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__r
      * its latency overlaps the neighbour loads; loaded after it, each
      * wavefront waits for one more memory round trip.  It keeps the six
      * words live across the staple.  Measured, alternating on one box: the
-     * FP64-link single-lattice sweep (capped at 4 waves per SIMD anyway,
-     * 80 -> 92 VGPRs) gains 2-4 % per launch at 128^3; the uncapped
-     * FP32-link one would go from 74 to 82 VGPRs, 6 to 5 waves, and lose
-     * 3 %; the batched sweep gains 4.5 % with FP32 links and loses 1.6 %
-     * with FP64 links.  The callers choose accordingly.  (Issuing all
-     * twelve neighbour loads before the first product gained nothing more
-     * and cost FP32-link over-relaxation 5 %.) */
+     * FP64-link single-lattice sweep (capped at 4 waves per SIMD anyway)
+     * gains 2-4 % per launch at 128^3; the uncapped FP32-link one is within
+     * 1 % either way and keeps the late load; the batched sweep gains 4.5 %
+     * with FP32 links and loses 1.6-2 % with FP64 links.  The callers choose
+     * accordingly.  (Issuing all twelve neighbour loads before the first
+     * product gained nothing more and cost FP32-link over-relaxation 11 %.) */
     uint32_t s[6];
     if constexpr (EARLY_RNG) {
         if (in_range) {
