@@ -199,7 +199,7 @@ template <int PMC, typename LinkT>
 __device__ static inline void en_av_block(const su2_bdesc &d, uint32_t b)
 {
     const uint32_t p = b & 1u;
-    const uint32_t h = (b >> 1) * blockDim.x + threadIdx.x;
+    const uint32_t h = (b >> 1) * SU2_EN_BLOCK + threadIdx.x;
     double e = 0.0, pl = 0.0;
     if (h < d.g.Vh) {
         const uint32_t q = su2_div(h, d.g.div_lyh);
@@ -208,7 +208,7 @@ __device__ static inline void en_av_block(const su2_bdesc &d, uint32_t b)
         const uint32_t x = q - t * d.g.Lx;
         su2_site_plaquettes<PMC, LinkT>((const LinkT *)d.U, d.g, p, h, t, x, yh, &e, &pl);
     }
-    su2_block_sum2(e, pl);
+    su2_block_sum2(e, pl, SU2_EN_BLOCK);
     if (threadIdx.x == 0) {
         d.part[2 * b] = e;
         d.part[2 * b + 1] = pl;
@@ -259,10 +259,10 @@ __global__ __launch_bounds__(1024) void k_ben_av_final(const su2_bhead *__restri
         }
         const uint32_t m = (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor;
         double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
-        for (uint32_t i = threadIdx.x; i < d.en_nb; i += 4 * blockDim.x) {
+        for (uint32_t i = threadIdx.x; i < d.en_nb; i += 4 * SU2_EN_FINAL_BLOCK) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint32_t q = i + k * blockDim.x;
+                const uint32_t q = i + k * SU2_EN_FINAL_BLOCK;
                 if (q < d.en_nb) {
                     se[k] += d.part[2 * q];
                     sp[k] += d.part[2 * q + 1];
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(1024) void k_ben_av_final(const su2_bhead *__restri
         }
         double te = (se[0] + se[1]) + (se[2] + se[3]);
         double tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
-        su2_block_sum2(te, tp);
+        su2_block_sum2(te, tp, SU2_EN_FINAL_BLOCK);
         if (threadIdx.x == 0) {
             const double invV = 1.0 / (2.0 * (double)d.g.Vh);
             d.resE[m] += te * invV;
@@ -347,12 +347,12 @@ hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, co
                              uint32_t *ctr, uint32_t grid, uint32_t final_grid, hipStream_t st)
 {
     if (f32)
-        hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr);
+        hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(SU2_EN_BLOCK), 0, st, H, D, B, max_desc, ctr);
     else
-        hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(256), 0, st, H, D, B, max_desc, ctr);
+        hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(SU2_EN_BLOCK), 0, st, H, D, B, max_desc, ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ben_av_final, dim3(final_grid), dim3(1024), 0, st, H, D, ctr);
+    hipLaunchKernelGGL(k_ben_av_final, dim3(final_grid), dim3(SU2_EN_FINAL_BLOCK), 0, st, H, D, ctr);
     return hipGetLastError();
 }
 
@@ -362,4 +362,7 @@ hipError_t su2_launch_btick_end(const su2_bhead *H, const su2_bdesc *D, uint32_t
     return hipGetLastError();
 }
 
-uint32_t su2_en_av_logical_blocks(const su2_kgeom &g) { return su2_grid_blocks(2ull * ((g.Vh + 255) / 256) * 256, 256); }
+uint32_t su2_en_av_logical_blocks(const su2_kgeom &g)
+{
+    return su2_grid_blocks(2ull * ((g.Vh + SU2_EN_BLOCK - 1) / SU2_EN_BLOCK) * SU2_EN_BLOCK, SU2_EN_BLOCK);
+}

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_en_av_partial(const LinkT *__restrict__
 {
     const uint32_t b = su2_xcd_block();
     const uint32_t p = b & 1u;
-    const uint32_t h = (b >> 1) * blockDim.x + threadIdx.x;
+    const uint32_t h = (b >> 1) * SU2_EN_BLOCK + threadIdx.x;
     double e = 0.0, pl = 0.0;
     if (h < g.Vh) {
         const uint32_t q = su2_div(h, g.div_lyh);
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_en_av_partial(const LinkT *__restrict__
         const uint32_t x = q - t * g.Lx;
         su2_site_plaquettes<PMC, LinkT>(U, g, p, h, t, x, yh, &e, &pl);
     }
-    su2_block_sum2(e, pl);
+    su2_block_sum2(e, pl, SU2_EN_BLOCK);
     if (threadIdx.x == 0) {
         work[2 * b] = e;
         work[2 * b + 1] = pl;
@@ -103,17 +103,17 @@ __global__ __launch_bounds__(256) void k_en_av_partial(const LinkT *__restrict__
 }
 
 /* Pass 2: one block of 1024 threads adds the partials in a fixed order (four
- * accumulators per thread over a stride of 4 * blockDim, then the block
+ * accumulators per thread over a stride of 4 * 1024, then the block
  * sum) and accumulates sum / V into res, res2 (the reference adds into
  * zeroed slots with atomicAdd, functions/observables.h:137-140). */
 __global__ __launch_bounds__(1024) void k_en_av_final(const double *__restrict__ work, uint32_t n, double invV,
                                                       double *__restrict__ res, double *__restrict__ res2)
 {
     double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t i = threadIdx.x; i < n; i += 4 * blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < n; i += 4 * SU2_EN_FINAL_BLOCK) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint32_t j = i + k * blockDim.x;
+            const uint32_t j = i + k * SU2_EN_FINAL_BLOCK;
             if (j < n) {
                 se[k] += work[2 * j];
                 sp[k] += work[2 * j + 1];
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(1024) void k_en_av_final(const double *__restrict__
     }
     double te = (se[0] + se[1]) + (se[2] + se[3]);
     double tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
-    su2_block_sum2(te, tp);
+    su2_block_sum2(te, tp, SU2_EN_FINAL_BLOCK);
     if (threadIdx.x == 0) {
         *res += te * invV;
         *res2 += tp * invV;
@@ -247,8 +247,11 @@ hipError_t su2_launch_normalize(int f32, void *U, const su2_kgeom &g, hipStream_
 }
 
 /* en_av launch shape: one site per thread, two blocks (the two parities)
- * per chunk of 256 half-sites. */
-static uint32_t en_av_blocks(const su2_kgeom &g) { return su2_grid_blocks(2ull * ((g.Vh + 255) / 256) * 256, 256); }
+ * per chunk of SU2_EN_BLOCK half-sites. */
+static uint32_t en_av_blocks(const su2_kgeom &g)
+{
+    return su2_grid_blocks(2ull * ((g.Vh + SU2_EN_BLOCK - 1) / SU2_EN_BLOCK) * SU2_EN_BLOCK, SU2_EN_BLOCK);
+}
 
 /* partial sums, 2 per block */
 size_t su2_en_av_work_doubles(const su2_kgeom &g) { return 2 * (size_t)en_av_blocks(g); }
@@ -259,19 +262,19 @@ hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g,
     const uint32_t nb = en_av_blocks(g);
     if (f32) {
         if (pmc)
-            hipLaunchKernelGGL((k_en_av_partial<1, su2_l32>), dim3(nb), dim3(256), 0, st, (const su2_l32 *)U, g, work);
+            hipLaunchKernelGGL((k_en_av_partial<1, su2_l32>), dim3(nb), dim3(SU2_EN_BLOCK), 0, st, (const su2_l32 *)U, g, work);
         else
-            hipLaunchKernelGGL((k_en_av_partial<0, su2_l32>), dim3(nb), dim3(256), 0, st, (const su2_l32 *)U, g, work);
+            hipLaunchKernelGGL((k_en_av_partial<0, su2_l32>), dim3(nb), dim3(SU2_EN_BLOCK), 0, st, (const su2_l32 *)U, g, work);
     } else {
         if (pmc)
-            hipLaunchKernelGGL((k_en_av_partial<1, su2_l64>), dim3(nb), dim3(256), 0, st, (const su2_l64 *)U, g, work);
+            hipLaunchKernelGGL((k_en_av_partial<1, su2_l64>), dim3(nb), dim3(SU2_EN_BLOCK), 0, st, (const su2_l64 *)U, g, work);
         else
-            hipLaunchKernelGGL((k_en_av_partial<0, su2_l64>), dim3(nb), dim3(256), 0, st, (const su2_l64 *)U, g, work);
+            hipLaunchKernelGGL((k_en_av_partial<0, su2_l64>), dim3(nb), dim3(SU2_EN_BLOCK), 0, st, (const su2_l64 *)U, g, work);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const double invV = 1.0 / (2.0 * (double)g.Vh);
-    hipLaunchKernelGGL(k_en_av_final, dim3(1), dim3(1024), 0, st, (const double *)work, nb, invV, res, res2);
+    hipLaunchKernelGGL(k_en_av_final, dim3(1), dim3(SU2_EN_FINAL_BLOCK), 0, st, (const double *)work, nb, invV, res, res2);
     return hipGetLastError();
 }
 

@@ -54,9 +54,17 @@ __device__ static inline void su2_site_plaquettes(const LinkT *__restrict__ U, c
     *pl = (3.0 - a - b - c) / 3.0;
 }
 
-/* Deterministic block sum of two doubles: wavefront shuffles in a fixed
- * butterfly, then the wavefront totals in order (result in thread 0). */
-__device__ static inline void su2_block_sum2(double &u, double &v)
+/* Workgroup size of every en_av pass-1 launch (single-lattice and batched):
+ * logical block b covers half-sites [(b >> 1) * SU2_EN_BLOCK, ...) of parity
+ * b & 1, which fixes the partials and their summation order. */
+#define SU2_EN_BLOCK 256u
+/* Workgroup size of every en_av pass-2 launch. */
+#define SU2_EN_FINAL_BLOCK 1024u
+
+/* Deterministic block sum of two doubles over a workgroup of nthreads (the
+ * launch's blockDim.x, a multiple of 64 up to 1024): wavefront shuffles in a
+ * fixed butterfly, then the wavefront totals in order (result in thread 0). */
+__device__ static inline void su2_block_sum2(double &u, double &v, uint32_t nthreads)
 {
     __shared__ double sh[2][16];
 #pragma unroll
@@ -65,7 +73,7 @@ __device__ static inline void su2_block_sum2(double &u, double &v)
         v += __shfl_xor(v, off, 64);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nw = (blockDim.x + 63) >> 6;
+    const int nw = (int)((nthreads + 63u) >> 6);
     if (lane == 0) {
         sh[0][wave] = u;
         sh[1][wave] = v;

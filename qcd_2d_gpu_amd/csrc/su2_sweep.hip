@@ -74,9 +74,12 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
                                                                         int *__restrict__ accept, const su2_kgeom &g,
                                                                         const su2_kphys &ph, int parity,
                                                                         uint32_t order, uint32_t strip,
-                                                                        uint32_t bpp, int acc_mode)
+                                                                        uint32_t bpp, uint32_t nthr, int acc_mode)
 {
-    const uint32_t h = sweep_block(order, strip, bpp) * blockDim.x + threadIdx.x;
+    /* nthr = blockDim.x, passed as an argument: blockDim.x itself is read
+     * from the dispatch packet with a vector load whose wait would sit in
+     * front of every wavefront's first link and RNG loads */
+    const uint32_t h = sweep_block(order, strip, bpp) * nthr + threadIdx.x;
     if constexpr (UPD == SU2_K_HB) {
         __shared__ su2_kp_queue q;
         /* FP64 links: early RNG load (see su2_sweep_site_hb_wg) */
@@ -91,9 +94,10 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
 template <int MU, int BC, int UPD, typename LinkT>
 __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *__restrict__ rng,
                                                int *__restrict__ accept, su2_kgeom g, su2_kphys ph, int parity,
-                                               uint32_t order, uint32_t strip, uint32_t bpp, int acc_mode)
+                                               uint32_t order, uint32_t strip, uint32_t bpp, uint32_t nthr,
+                                               int acc_mode)
 {
-    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp, acc_mode);
+    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr, acc_mode);
 }
 
 /* FP64-link sweeps run at most 4 waves per SIMD.  More workgroups in flight
@@ -108,9 +112,9 @@ __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *
 template <int MU, int BC, int UPD, typename LinkT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void
 k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
-           int parity, uint32_t order, uint32_t strip, uint32_t bpp, int acc_mode)
+           int parity, uint32_t order, uint32_t strip, uint32_t bpp, uint32_t nthr, int acc_mode)
 {
-    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp, acc_mode);
+    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr, acc_mode);
 }
 
 template <int MU, int BC, int UPD, typename LinkT>
@@ -132,10 +136,10 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
     }
     if constexpr (sizeof(LinkT) == sizeof(su2_l64))
         hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
-                           p, parity, order, strip, bpp, acc_mode);
+                           p, parity, order, strip, bpp, (uint32_t)block, acc_mode);
     else
         hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
-                           p, parity, order, strip, bpp, acc_mode);
+                           p, parity, order, strip, bpp, (uint32_t)block, acc_mode);
     return hipGetLastError();
 }
 
