@@ -111,12 +111,20 @@ __global__ __launch_bounds__(1024) void k_en_av_final(const double *__restrict__
 {
     double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t i = threadIdx.x; i < n; i += 4 * SU2_EN_FINAL_BLOCK) {
+        /* the four loads first (an index past n reads partial 0 and is not
+         * added), so their latencies overlap */
+        double ve[4], vp[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint32_t j = i + k * SU2_EN_FINAL_BLOCK;
-            if (j < n) {
-                se[k] += work[2 * j];
-                sp[k] += work[2 * j + 1];
+            const uint32_t j = i + k * SU2_EN_FINAL_BLOCK, jj = j < n ? j : 0u;
+            ve[k] = work[2 * jj];
+            vp[k] = work[2 * jj + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i + k * SU2_EN_FINAL_BLOCK < n) {
+                se[k] += ve[k];
+                sp[k] += vp[k];
             }
         }
     }
