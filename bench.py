@@ -6,8 +6,10 @@ periodic, heat-bath followed by one over-relaxation sweep, beta 5.0, hot
 start with Haar-random links from seed 7, FP64 links.  One step = one
 heat-bath sweep (6 launches) + one over-relaxation sweep (6 launches) + the
 fused energy/plaquette measurement, with the reunitarisation every 100
-steps, as in the reference's measurement loop (kernel.cu:192-208).  Warm-up
-steps play the role of thermalisation.
+steps, as in the reference's measurement loop (kernel.cu:192-208).  Before
+anything is timed the lattice is thermalised with the configuration's 200
+heat-bath + over-relaxation sweeps (--therm, the thermalisation loop of
+kernel.cu:161-179); --warmup steps after that are a timing warm-up only.
 
 Prints ONE JSON line.  value = link updates per second of the whole step
 (2 x 3 V updates per step).  roofline: the heat-bath sweep kernel, 240
@@ -44,6 +46,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--therm", type=int, default=200,
+                    help="thermalisation sweeps (heat-bath + over-relaxation) before any timing, whatever "
+                         "--warmup says")
     ap.add_argument("--L", type=int, default=128, help="Lt = Lx = Ly")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64 links, 1 FP32 links")
     ap.add_argument("--cpu-steps", type=int, default=-1, help="CPU baseline steps (-1: auto, 0: skip)")
@@ -62,6 +67,8 @@ def parse():
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-ensemble comparison at --precision and print its JSON")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving stream and print its JSON")
+    ap.add_argument("--ceiling-only", action="store_true",
+                    help="run only the streaming-bandwidth ceilings (su2_stream_probe) and print their JSON")
     ap.add_argument("--serve-requests", type=int, default=100, help="served requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-burst", type=int, default=200,
@@ -100,6 +107,14 @@ def gpu_bench(args):
         if res_slot is not None:
             g.en_av_into(False, res.value + 16 * res_slot, res.value + 16 * res_slot + 8)
 
+    # thermalisation (kernel.cu:161-179): reunitarise before the sweep every
+    # 100 sweeps and once after the loop
+    for i in range(1, args.therm + 1):
+        if i % 100 == 0:
+            g.normalize()
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
+    g.normalize()
     for i in range(1, args.warmup + 1):
         step(i, 0)
     g.synchronize()
@@ -195,10 +210,26 @@ def gpu_bench(args):
                 en_ms=float(np.mean(en_ms)), me_ms=float(np.mean(me_ms[1:] if len(me_ms) > 1 else me_ms)))
 
 
+def host_cores():
+    """Host cores: os.cpu_count() (nproc) and the cores this process may run
+    on (its affinity mask)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "affinity": aff}
+
+
+def host_threads(args):
+    """Threads of the checker's OpenMP loops: every core this process may
+    run on, unless --cpu-threads says otherwise."""
+    return args.cpu_threads or host_cores()["affinity"]
+
+
 def cpu_baseline(args, V_gpu):
     from oracle import oracle as O
 
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    threads = host_threads(args)
     O.set_threads(threads)
     L = args.L
     o = O.Oracle(L, L, L, 5.0)
@@ -214,7 +245,7 @@ def cpu_baseline(args, V_gpu):
         return None
     sec = o.time_sweeps(0, 1, 1, steps)
     ups = 2 * 3 * o.V * steps / sec
-    out = {"value": ups, "unit": "link updates/s", "cores": threads, "kind": "port",
+    out = {"value": ups, "unit": "link updates/s", "cores": threads, "kind": "port", "host": host_cores(),
            "sample": f"{steps} steps (heat-bath + over-relaxation sweep) of the same {L}^3 lattice, "
                      f"{sec:.1f} s, OpenMP C restatement of the reference"}
     if threads > 1 and args.cpu_steps < 0:
@@ -314,7 +345,7 @@ def small_config_bench(args):
     import qcd_2d_gpu_amd as q
     from oracle import oracle as O
 
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    threads = host_threads(args)
     O.set_threads(threads)
     n_t = n_m = 100
     out = {}
@@ -419,7 +450,7 @@ def ensemble_cpu_sample(args, jobs):
     ensemble's CPU rate."""
     from oracle import oracle as O
 
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    threads = host_threads(args)
     O.set_threads(threads)
     sec, ups = 0.0, 0
     for j in jobs:
@@ -523,29 +554,106 @@ def fp32_check(args):
     return out
 
 
+NCU = 256  # compute units of an MI355X
+
+
+def stream_ceiling(args):
+    """Measured streaming ceilings that the sweep's HBM rate is compared
+    with: su2_stream_probe (reads source arrays and writes their XOR, 16-B
+    items, nothing else to do) over unroll x nontemporal x grid, best rate
+    per (regime, reads).  'dram': 512 MiB per array, far beyond the 256 MB
+    Infinity Cache; 'cache': 40 MiB per array (at most 200 MiB in all),
+    resident in it.  reads = 1 is a copy; reads = 4 is close to the heat-bath
+    sweep's 3.3 : 1 read : write byte ratio.  Rate = (reads + 1) x bytes per
+    array / launch time; the launch time is one HIP-event bracket around 10
+    back-to-back launches / 10, after one warm-up launch, so the dispatch
+    gap is amortised as in a sweep of back-to-back launches."""
+    import qcd_2d_gpu_amd as q
+    from qcd_2d_gpu_amd import _lib as L
+
+    lib = q.load_library()
+    stream = C.c_void_p()
+    L.check(lib.su2_stream_create(C.byref(stream)), "stream_create")
+    a, b = C.c_void_p(), C.c_void_p()
+    L.check(lib.su2_event_create(C.byref(a)), "event")
+    L.check(lib.su2_event_create(C.byref(b)), "event")
+    ms = C.c_float()
+    out = {}
+    for regime, region in (("dram", 512 << 20), ("cache", 40 << 20)):
+        for reads in (1, 4):
+            src, dst = q.DeviceBuffer(reads * region), q.DeviceBuffer(region)
+            src.zero(stream)
+            dst.zero(stream)
+            rows = []
+            for unroll in (1, 2, 4, 8):
+                for nt in (0, 1):
+                    for bpc in (0, 2, 4, 8, 16, 32):
+                        blocks = bpc * NCU
+                        L.check(lib.su2_stream_probe(dst.ptr, src.ptr, region, reads, unroll, nt, blocks, stream),
+                                "stream_probe")
+                        L.check(lib.su2_event_record(a, stream), "record")
+                        for _ in range(10):
+                            L.check(lib.su2_stream_probe(dst.ptr, src.ptr, region, reads, unroll, nt, blocks,
+                                                         stream), "stream_probe")
+                        L.check(lib.su2_event_record(b, stream), "record")
+                        L.check(lib.su2_event_elapsed_ms(a, b, C.byref(ms)), "elapsed")
+                        sec = ms.value * 1e-3 / 10
+                        rows.append({"unroll": unroll, "nontemporal": nt, "blocks_per_cu": bpc,
+                                     "gbs": (reads + 1) * region / sec / 1e9})
+            src.free()
+            dst.free()
+            best = max(rows, key=lambda r: r["gbs"])
+            out[f"{regime}_reads{reads}"] = {"bytes_per_array": region, "best_gbs": best["gbs"], "best": best,
+                                             "frac_of_hbm_peak": best["gbs"] / HBM_PEAK_GBS,
+                                             "worst_gbs": min(r["gbs"] for r in rows)}
+    lib.su2_event_destroy(a)
+    lib.su2_event_destroy(b)
+    lib.su2_stream_destroy(stream)
+    return out
+
+
+# The sources the sweep, en_av and copy kernels are compiled from.  A
+# rocprofv3 summary under profiles/ records their digest on its line
+# "kernel_sources_sha16: ..."; a figure read from a summary whose digest
+# differs from the current sources' is stale and not reported.
+KERNEL_SOURCES = ("su2_sweep.hip", "su2_sweep_site.h", "su2_update.h", "su2_quat.h", "su2_fmath.h",
+                  "su2_mrg32k3a.h", "su2_internal.h", "su2_misc.hip", "su2_observe.h")
+
+
+def kernel_sources_digest():
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "qcd_2d_gpu_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
 def load_profile_value(fname, key):
     """The number on the line "key: N" of a committed rocprofv3 summary
-    under profiles/, or None."""
+    under profiles/, or None when the summary is missing, lacks the key, or
+    was made from other kernel sources than the current ones."""
     p = os.path.join(ROOT, "profiles", fname)
-    if os.path.exists(p):
-        with open(p) as f:
-            for line in f:
-                if line.startswith(key + ":"):
-                    return float(line.split(":", 1)[1])
-    return None
+    if not os.path.exists(p):
+        return None
+    vals = {}
+    with open(p) as f:
+        for line in f:
+            if ":" in line and not line.startswith(" "):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    if vals.get("kernel_sources_sha16") != kernel_sources_digest() or key not in vals:
+        return None
+    return float(vals[key])
 
 
 def load_traffic():
     """HBM bytes per heat-bath launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present:
-    the line "bytes_per_launch: N" of profiles/hb_sweep_128cube_traffic.txt."""
-    p = os.path.join(ROOT, "profiles", "hb_sweep_128cube_traffic.txt")
-    if os.path.exists(p):
-        with open(p) as f:
-            for line in f:
-                if line.startswith("bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
-    return None
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated): the line
+    "bytes_per_launch: N" of profiles/hb_sweep_128cube_traffic.txt, if it
+    was made from the current kernel sources."""
+    return load_profile_value("hb_sweep_128cube_traffic.txt", "bytes_per_launch")
 
 
 def fp64_rate(r, updates):
@@ -574,6 +682,9 @@ def main():
         r = batched_bench(args, args.precision)
         r.pop("_plaquette")
         print(json.dumps(r))
+        return
+    if args.ceiling_only:
+        print(json.dumps({"stream_ceiling": stream_ceiling(args)}))
         return
     if args.serving_only:
         out = {"serving": serving_bench(args)}
@@ -648,7 +759,10 @@ def main():
                      "kernel": "heat-bath sweep launch (one direction x parity)",
                      "bytes_per_update": BYTES_HB, "updates_per_launch": per_launch_updates,
                      "avg_launch_us": r["hb_ms"] * 1e3,
-                     "traffic_gbs": (load_traffic() / (r["hb_ms"] * 1e-3) / 1e9) if load_traffic() else None},
+                     "traffic_gbs": (load_traffic() / (r["hb_ms"] * 1e-3) / 1e9) if load_traffic() else None,
+                     "traffic_source": "profiles/hb_sweep_128cube_traffic.txt" if load_traffic() else
+                     "none: no rocprofv3 summary made from the current kernel sources "
+                     f"({kernel_sources_digest()})"},
         "overrelax_launch": {"avg_launch_us": r["or_ms"] * 1e3, "achieved_gbs": or_gbs,
                              "updates_per_s_kernel": per_launch_updates / (r["or_ms"] * 1e-3)},
         "fp64_rate": fp64_rate(r, per_launch_updates),

@@ -195,6 +195,15 @@ int su2_memset_zero(void *dst, size_t bytes, void *stream);
 /* Device-to-device copy by a plain streaming kernel (16 B per thread); also
  * the reference kernel for calibrating hardware byte counters. */
 int su2_device_copy(void *dst, const void *src, size_t bytes, void *stream);
+/* Extension (no counterpart in the reference): streaming-bandwidth probe,
+ * the measured ceiling a sweep's HBM rate is compared with.  dst[i] = XOR of
+ * src[r * bytes + i] over r < reads (reads 1, 2, 4 or 5; reads = 1 is a
+ * copy), 16-byte items; src holds reads regions of `bytes` each.  Every
+ * thread keeps `unroll` (1, 2, 4, 8) items of each source in flight;
+ * nontemporal != 0 marks loads and stores nontemporal; blocks = workgroups
+ * of 256 (0: one item per thread per unroll, at most 2^20). */
+int su2_stream_probe(void *dst, const void *src, size_t bytes, int reads, int unroll, int nontemporal, int blocks,
+                     void *stream);
 int su2_stream_create(void **stream);
 int su2_stream_destroy(void *stream);
 int su2_stream_synchronize(void *stream);

@@ -417,6 +417,20 @@ int su2_device_copy(void *dst, const void *src, size_t bytes, void *stream)
     return e == hipSuccess ? SU2_OK : hip_fail(e, "copy launch");
 }
 
+int su2_stream_probe(void *dst, const void *src, size_t bytes, int reads, int unroll, int nontemporal, int blocks,
+                     void *stream)
+{
+    if (bytes == 0) return SU2_OK;
+    if (!dst || !src || (bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+        return fail(SU2_EINVAL, "stream_probe needs 16-byte aligned pointers and a size multiple of 16");
+    if (!(reads == 1 || reads == 2 || reads == 4 || reads == 5) ||
+        !(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8) || blocks < 0)
+        return fail(SU2_EINVAL, "stream_probe: reads 1, 2, 4 or 5; unroll 1, 2, 4 or 8; blocks >= 0");
+    hipError_t e = su2_launch_stream_probe(dst, src, bytes, reads, unroll, nontemporal != 0, (uint32_t)blocks,
+                                           S(stream));
+    return e == hipSuccess ? SU2_OK : hip_fail(e, "stream probe launch");
+}
+
 int su2_stream_create(void **stream)
 {
     if (!stream) return fail(SU2_EINVAL, "NULL");

@@ -151,6 +151,8 @@ hipError_t su2_launch_to_ref(int f32, const void *U, double *phi_ref, const su2_
 hipError_t su2_launch_from_ref(int f32, const double *phi_ref, void *U, const su2_kgeom &g, hipStream_t st);
 hipError_t su2_launch_fill_i32(int *p, int value, size_t n, hipStream_t st);
 hipError_t su2_launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+hipError_t su2_launch_stream_probe(void *dst, const void *src, size_t bytes, int reads, int unroll, int nt,
+                                   uint32_t blocks, hipStream_t st);
 
 /* su2_set_error (defined in su2_capi.cpp), shared with the C host driver */
 #include "su2_host.h"

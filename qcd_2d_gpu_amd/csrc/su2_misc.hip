@@ -180,6 +180,54 @@ __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const u
         dst[i] = src[i];
 }
 
+/* Bandwidth probe: dst[i] = src_0[i] ^ ... ^ src_{READS-1}[i] over READS
+ * source regions of n 16-byte items each, laid out one after the other in
+ * src (READS = 1 is a copy).  Each thread keeps UNROLL items of every
+ * source in flight (all loads issued before the first XOR); a wavefront's
+ * load of one item index is one contiguous 1 KB run.  NT marks loads and
+ * stores nontemporal.  This is the ceiling for a kernel that streams READS
+ * arrays in and one out with nothing else to do. */
+typedef unsigned int su2_v4u __attribute__((ext_vector_type(4)));
+
+template <int NT> __device__ static inline __attribute__((always_inline)) su2_v4u probe_load(const su2_v4u *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+template <int NT> __device__ static inline __attribute__((always_inline)) void probe_store(su2_v4u *p, su2_v4u v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+template <int READS, int UNROLL, int NT>
+__global__ __launch_bounds__(256) void k_stream_probe(su2_v4u *__restrict__ dst, const su2_v4u *__restrict__ src,
+                                                      size_t n)
+{
+    const size_t step = (size_t)gridDim.x * (256 * UNROLL);
+    for (size_t base = (size_t)blockIdx.x * (256 * UNROLL) + threadIdx.x; base < n; base += step) {
+        su2_v4u v[READS][UNROLL];
+#pragma unroll
+        for (int r = 0; r < READS; ++r)
+#pragma unroll
+            for (int k = 0; k < UNROLL; ++k) {
+                const size_t i = base + (size_t)k * 256;
+                v[r][k] = i < n ? probe_load<NT>(src + (size_t)r * n + i) : su2_v4u{0u, 0u, 0u, 0u};
+            }
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            su2_v4u a = v[0][k];
+#pragma unroll
+            for (int r = 1; r < READS; ++r) a ^= v[r][k];
+            const size_t i = base + (size_t)k * 256;
+            if (i < n) probe_store<NT>(dst + i, a);
+        }
+    }
+}
+
 /* The jump tables go to each device's __constant__ memory once; set-up may
  * be called from several host threads (a server thread and its caller), so
  * the check and the upload are serialised. */
@@ -322,4 +370,47 @@ hipError_t su2_launch_copy(void *dst, const void *src, size_t bytes, hipStream_t
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(k_copy16, dim3((uint32_t)nb), dim3(256), 0, st, (uint4 *)dst, (const uint4 *)src, n);
     return hipGetLastError();
+}
+
+template <int READS, int UNROLL>
+static hipError_t launch_probe_t(void *dst, const void *src, size_t n, int nt, uint32_t blocks, hipStream_t st)
+{
+    if (nt)
+        hipLaunchKernelGGL((k_stream_probe<READS, UNROLL, 1>), dim3(blocks), dim3(256), 0, st, (su2_v4u *)dst,
+                           (const su2_v4u *)src, n);
+    else
+        hipLaunchKernelGGL((k_stream_probe<READS, UNROLL, 0>), dim3(blocks), dim3(256), 0, st, (su2_v4u *)dst,
+                           (const su2_v4u *)src, n);
+    return hipGetLastError();
+}
+
+template <int READS>
+static hipError_t launch_probe_r(void *dst, const void *src, size_t n, int unroll, int nt, uint32_t blocks,
+                                 hipStream_t st)
+{
+    switch (unroll) {
+    case 1: return launch_probe_t<READS, 1>(dst, src, n, nt, blocks, st);
+    case 2: return launch_probe_t<READS, 2>(dst, src, n, nt, blocks, st);
+    case 4: return launch_probe_t<READS, 4>(dst, src, n, nt, blocks, st);
+    case 8: return launch_probe_t<READS, 8>(dst, src, n, nt, blocks, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t su2_launch_stream_probe(void *dst, const void *src, size_t bytes, int reads, int unroll, int nt,
+                                   uint32_t blocks, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    if (blocks == 0) {
+        const size_t nb = (n + 256 * (size_t)unroll - 1) / (256 * (size_t)unroll);
+        blocks = (uint32_t)(nb < ((size_t)1 << 20) ? nb : ((size_t)1 << 20));
+    }
+    switch (reads) {
+    case 1: return launch_probe_r<1>(dst, src, n, unroll, nt, blocks, st);
+    case 2: return launch_probe_r<2>(dst, src, n, unroll, nt, blocks, st);
+    case 4: return launch_probe_r<4>(dst, src, n, unroll, nt, blocks, st);
+    case 5: return launch_probe_r<5>(dst, src, n, unroll, nt, blocks, st);
+    }
+    return hipErrorInvalidValue;
 }

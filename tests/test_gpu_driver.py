@@ -247,3 +247,32 @@ def test_runs_are_deterministic(gpu):
     (f1, s1, a1, o1), (f2, s2, a2, o2) = runs
     assert np.array_equal(f1.view(np.uint64), f2.view(np.uint64))
     assert np.array_equal(s1, s2) and np.array_equal(a1, a2) and o1 == o2
+
+
+@pytest.mark.slow
+def test_c2_long_run_matches_checker(gpu, oracle_mod, tmp_path):
+    """The large single-lattice configuration on its own schedule: 128^3,
+    periodic, heat-bath + one over-relaxation sweep per sweep, beta 5.0,
+    Haar-random hot start from seed 7, FP64, 200 thermalisation + 100
+    measured sweeps (kernel.cu:161-236), through su2_run.  Every one of the
+    100 energies and plaquettes agrees with the checker's run loop to 1e-12
+    relative, the acceptance exactly, and the binned means (10 bins) agree
+    far inside their binned errors."""
+    from _jobs import close as close_rel, oracle_job
+
+    q = gpu
+    print("[driver] C2 128^3 HB+OR beta 5 Haar seed 7, 200 + 100 sweeps", flush=True)
+    job = q.Job(128, 128, 128, 5.0, bc=0, algo=q.ALGO_HEATBATH, start=q.START_HAAR, seed=7, n_therm=200,
+                n_meas=100, or_sweeps=1)
+    r = q.run(Lt=128, Lx=128, Ly=128, beta=5.0, boundary=0, algo=q.ALGO_HEATBATH, init=q.START_HAAR, N=100,
+              N_term=200, or_sweeps=1, seed=7, save=1, sconf=1, outdir=str(tmp_path))
+    print("[driver]   GPU run done; checker run loop", flush=True)
+    E, P, _, _, acc = oracle_job(oracle_mod, job, 0)
+    assert close_rel(r.energy, E) and close_rel(r.plaquette, P)
+    assert abs(r.accept - acc) <= 1e-12 and abs(acc - 1.0) <= 1e-12
+    for gpu_series, cpu_series in ((r.energy, E), (r.plaquette, P)):
+        mg, eg = q.stats.binned_error(gpu_series, 10)
+        mc, ec = q.stats.binned_error(cpu_series, 10)
+        assert eg > 0 and abs(mg - mc) <= 1e-12 * abs(mc) and abs(mg - mc) < 0.01 * eg
+    # the thermalised plaquette of the weak-coupling side: 1 - <P> close to 1/beta (3-D SU(2))
+    assert 0.75 < r.P_mean < 0.85

@@ -8,6 +8,8 @@ import time
 import numpy as np
 import pytest
 
+from _jobs import check_against_oracle
+
 pytestmark = pytest.mark.gpu
 
 
@@ -71,7 +73,10 @@ def mixed_jobs(q, n, seed0=1000, small=True):
     return jobs
 
 
-def check_same(q, eng, slot, job, prec, tag):
+def check_same(q, eng, slot, job, prec, tag, oracle=None):
+    """The job's engine results bit for bit against the same job run alone
+    on the single-lattice entry points; with `oracle` (the checker module)
+    also against the checker's run loop."""
     got = eng.read(slot, field=True, rng=True)
     E, P, phi, st, acc = run_alone(q, job, prec)
     assert np.array_equal(got["E"].view(np.uint64), E.view(np.uint64)), f"{tag}: energies differ"
@@ -79,11 +84,19 @@ def check_same(q, eng, slot, job, prec, tag):
     assert np.array_equal(got["field"].view(np.uint64), phi.view(np.uint64)), f"{tag}: links differ"
     assert np.array_equal(got["rng"], st), f"{tag}: RNG states differ"
     assert abs(got["accept"] - acc) <= 1e-12 * max(1.0, abs(acc)), f"{tag}: acceptance {got['accept']} vs {acc}"
+    if oracle is not None:
+        check_against_oracle(oracle, job, prec, got["E"], got["P"], got["field"], got["rng"], got["accept"],
+                             tag=f"{tag} vs checker")
 
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_equals_alone(gpu, prec, use_graph):
+def test_batch_equals_alone(gpu, oracle_mod, prec, use_graph):
+    """Ten mixed jobs (every boundary, algorithm, start, over-relaxation
+    count and reunitarisation cadence) in one engine: each job's series,
+    final links, RNG states and acceptance equal, bit for bit, the same job
+    run alone, and agree with the checker's run loop (kernel.cu:161-208):
+    links and RNG states bit for bit, series to 1e-12."""
     q = gpu
     _say(f"[engine] batch vs alone prec={prec} graph={use_graph}")
     jobs = mixed_jobs(q, 10, seed0=2000 + 10 * prec)
@@ -94,7 +107,7 @@ def test_batch_equals_alone(gpu, prec, use_graph):
         for s, j in zip(slots, jobs):
             d, t = eng.status(s)
             assert d == t == j.n_therm + j.n_meas * j.n_cor
-            check_same(q, eng, s, j, prec, f"slot {s} {j}")
+            check_same(q, eng, s, j, prec, f"slot {s} {j}", oracle=oracle_mod)
     finally:
         eng.close()
 
@@ -214,12 +227,14 @@ def test_server_rejects_invalid_and_oversized(gpu):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_ensemble_of_64_at_full_size(gpu, prec):
+def test_ensemble_of_64_at_full_size(gpu, oracle_mod, prec):
     """The measured ensemble itself (64 lattices, 1.89 M sites, Lt = Lx over
     even 8..48, Ly over even 8..64, beta in [2, 8], every boundary, seeds
     1000 + i, 20 + 80 sweeps) in one engine with graphs and the autotuned
     shape: every job's 80 energies and plaquettes equal, bit for bit, those
-    of the same job run alone through the C driver su2_run."""
+    of the same job run alone through the C driver su2_run; four jobs (the
+    smallest of each boundary, and the largest) agree with the checker's
+    run loop: links and RNG states bit for bit, series to 1e-12."""
     import ctypes as C
     import os
 
@@ -243,5 +258,80 @@ def test_ensemble_of_64_at_full_size(gpu, prec):
                       N=j.n_meas, N_term=j.n_therm, seed=j.seed, precision=prec, save=1, sconf=1)
             assert np.array_equal(got["E"].view(np.uint64), r.energy.view(np.uint64)), f"job {s}: energies"
             assert np.array_equal(got["P"].view(np.uint64), r.plaquette.view(np.uint64)), f"job {s}: plaquettes"
+        picks = {min((j.V, k) for k, j in enumerate(jobs) if j.bc == bc)[1] for bc in (0, 1, 2)}
+        picks.add(max((j.V, k) for k, j in enumerate(jobs))[1])
+        for k in sorted(picks):
+            j = jobs[k]
+            _say(f"[engine]   job {k} {j.Lt}x{j.Lx}x{j.Ly} bc={j.bc} against the checker")
+            got = eng.read(slots[k], field=True, rng=True)
+            check_against_oracle(oracle_mod, j, prec, got["E"], got["P"], got["field"], got["rng"], got["accept"],
+                                 tag=f"ensemble job {k}")
     finally:
         eng.close()
+
+
+def c5_requests(q, n, rng):
+    """Run requests of the serving configuration: Lt = Lx in {16, 24, 32,
+    48, 64}, Ly in {16, 32, 64}, beta uniform in [2, 8] (rounded to float as
+    the reference stores it), boundary uniform over periodic / PCC / PMC,
+    heat-bath, hot start, 50 thermalisation + 200 measured sweeps."""
+    L = rng.choice([16, 24, 32, 48, 64], n)
+    Ly = rng.choice([16, 32, 64], n)
+    beta = rng.uniform(2.0, 8.0, n)
+    bc = rng.integers(0, 3, n)
+    return [q.Job(int(L[i]), int(L[i]), int(Ly[i]), float(np.float32(beta[i])), bc=int(bc[i]),
+                  algo=q.ALGO_HEATBATH, start=q.START_HOT, seed=9000 + i, n_therm=50, n_meas=200)
+            for i in range(n)]
+
+
+def test_server_c5_poisson_stream(gpu, oracle_mod):
+    """The serving configuration as a stream: 40 requests of the C5 shape
+    arriving as a Poisson process (mean 200/s, so that several are in
+    flight at once) into a server whose memory budget holds about three of
+    the largest jobs, so that requests wait for memory and join running
+    ticks.  Every served series equals, bit for bit, the same job run alone
+    through the C driver su2_run (the reference's run loop,
+    kernel.cu:161-236), and the four smallest jobs agree with the checker's
+    run loop to 1e-12 per measurement."""
+    import ctypes as C
+
+    q = gpu
+    _say("[server] C5 Poisson stream, 40 requests, tight budget")
+    rng = np.random.default_rng(777)
+    jobs = c5_requests(q, 40, rng)
+    arrivals = np.cumsum(rng.exponential(1.0 / 200.0, size=len(jobs)))
+    lib = q.load_library()
+    big = max(lib.su2_job_bytes(C.byref(j.c()), 0) for j in jobs)
+    srv = q.Server(precision=0, max_jobs=64, budget_bytes=3 * big + (1 << 20), use_graph=True)
+    try:
+        base = srv.now()
+        ids = []
+        for t, j in zip(arrivals, jobs):
+            d = base + t - srv.now()
+            if d > 0:
+                time.sleep(d)
+            ids.append(srv.submit(j))
+        srv.drain()
+        series = {i: srv.series(i) for i in ids}
+        res = {r.id: r for r in srv.poll()}
+        assert sorted(res) == sorted(ids)
+        for i, j in zip(ids, jobs):
+            assert res[i].status == 0, f"request {i}: status {res[i].status}"
+            r = q.run(Lt=j.Lt, Lx=j.Lx, Ly=j.Ly, beta=j.beta, boundary=j.bc, algo=j.algo, init=q.START_HOT,
+                      N=j.n_meas, N_term=j.n_therm, seed=j.seed, save=1, sconf=1)
+            assert np.array_equal(series[i][0].view(np.uint64), r.energy.view(np.uint64)), f"request {i}: energies"
+            assert np.array_equal(series[i][1].view(np.uint64), r.plaquette.view(np.uint64)), \
+                f"request {i}: plaquettes"
+            assert abs(res[i].P_mean - r.P_mean) <= 1e-12 and abs(res[i].accept - r.accept) <= 1e-12
+        for k in sorted(range(len(jobs)), key=lambda k: jobs[k].V)[:4]:
+            j = jobs[k]
+            _say(f"[server]   request {ids[k]} {j.Lt}x{j.Lx}x{j.Ly} bc={j.bc} against the checker")
+            E, P = series[ids[k]]
+            check_against_oracle(oracle_mod, j, 0, E, P, accept=res[ids[k]].accept, tag=f"served request {ids[k]}")
+        waited = [res[i].t_admit - res[i].t_submit for i in ids]
+        iv = sorted((res[i].t_admit, res[i].t_done) for i in ids)
+        _say(f"[server]   longest wait for admission {max(waited) * 1e3:.1f} ms")
+        assert any(nxt[0] < cur[1] for cur, nxt in zip(iv, iv[1:])), "no two requests were resident together"
+        assert max(waited) > 0.005, "no request waited for memory"
+    finally:
+        srv.close()
