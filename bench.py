@@ -630,11 +630,11 @@ def kernel_sources_digest():
     return h.hexdigest()[:16]
 
 
-def load_profile_value(fname, key):
+def load_profile_value(fname, key, profiles_dir=None):
     """The number on the line "key: N" of a committed rocprofv3 summary
     under profiles/, or None when the summary is missing, lacks the key, or
     was made from other kernel sources than the current ones."""
-    p = os.path.join(ROOT, "profiles", fname)
+    p = os.path.join(profiles_dir or os.path.join(ROOT, "profiles"), fname)
     if not os.path.exists(p):
         return None
     vals = {}

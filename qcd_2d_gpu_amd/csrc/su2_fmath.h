@@ -32,14 +32,35 @@
  *    polynomial of sin t on |t| <= pi/2 (through t^17/17!, truncation
  *    < 5e-14) with no data-dependent branch: on the GPU every lane of a
  *    wavefront runs the same instructions.
- *  - float versions evaluate the double function on the exactly converted
- *    argument and round once to float; with the truncations above they are
- *    within one float ulp (correctly rounded unless the exact value lies
- *    within ~1e-12 relative of a rounding boundary), the accuracy class of
- *    the library functions the reference calls.
- *  - sqrtf = (float)sqrt((double)x) is the correctly rounded float square
- *    root (53 >= 2*24+2, so the double rounding is innocuous); rsqrt(x) is
- *    1.0/sqrt(x), two correctly rounded operations.
+ *  - float versions (logf, cospif, sincospif: the heat-bath's) evaluate in
+ *    double arithmetic on the exactly converted argument, with polynomials
+ *    just accurate enough for a float result, and round once to float:
+ *      logf: x = 2^e m with m in [0x3f3504f3, 0x3fb504f2] (as floats, about
+ *        [sqrt(2)/2, sqrt(2))), split off the float's bits with integer
+ *        operations; log x = e ln2 + f (1 - f/2 + f^2 U(f)), f = m - 1 exact,
+ *        U the degree-8 Chebyshev interpolant of (log1p(f)/f - 1 + f/2)/f^2
+ *        with float coefficients, evaluated in float (each step one fused
+ *        multiply-add with a 32-bit literal); the rest in double.  U's term
+ *        is at most 7 % of log1p(f)/f, so its float rounding costs at most
+ *        about 0.3 ulp of the result; no division;
+ *      sinpi/cospi: the exact reduction above, then sin(pi d) = d R(d^2),
+ *        R the degree-5 Chebyshev interpolant of sin(pi d)/d in d^2 on
+ *        |d| <= 1/2 (relative error 2.7e-11).
+ *    The coefficients come from gen_fmath_coeffs.py (Chebyshev
+ *    interpolation in 60-digit arithmetic).  Every result is within one
+ *    float ulp of the exact value (the accuracy class of the library
+ *    functions the reference calls); the CPU tests check every float of the
+ *    heat-bath's argument ranges.  Double arithmetic issues at the float
+ *    rate on gfx950 (packed FP32 is not used), but a double constant that is
+ *    not 0, 0.5, 1, 2 or 4 costs two register moves per use where a float
+ *    constant is a free literal: hence float for the log's long tail.
+ *  - sqrtf is the IEEE (correctly rounded) float square root on both
+ *    targets.  rsqrt(x) = 1/sqrt(x) by four Newton steps
+ *    y <- y + y (1/2 - (x/2) y^2) from the classic bit-level seed (the
+ *    exponent halved and negated by an integer subtraction, relative error
+ *    < 3.5 %); quadratic convergence leaves the rounding of the last step,
+ *    a few ulp: three operations per step instead of a correctly rounded
+ *    square root and a division.
  *
  * The header is valid C99 (for the CPU checker) and HIP C++.
  */
@@ -92,12 +113,24 @@ SU2_FM_FN double su2_sqrt_rn(double x) { return __builtin_sqrt(x); }
 SU2_FM_FN double su2_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 SU2_FM_FN float su2_fmaf(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-/* rsqrt of the reference (functions/su2.h:5,17; mc_updates.h:58) as two
- * correctly rounded operations. */
-SU2_FM_FN double su2_rsqrt(double x) { return 1.0 / __builtin_sqrt(x); }
+/* rsqrt of the reference (functions/su2.h:5,17; mc_updates.h:58) for x > 0
+ * finite and normal (link norms and staple norms): four Newton steps from
+ * the bit-level seed; rsqrt(0) = +inf as 1/sqrt(0), so a zero staple still
+ * gives the NaN link it gives with exact arithmetic. */
+SU2_FM_FN double su2_rsqrt(double x)
+{
+    double y = su2_fm_from_bits(0x5fe6eb50c7b537a9ULL - (su2_fm_bits(x) >> 1));
+    const double h = 0.5 * x;
+    /* y (3/2 - h y^2) as y + y (1/2 - h y^2) */
+    y = su2_fma(y, su2_fma(-h * y, y, 0.5), y);
+    y = su2_fma(y, su2_fma(-h * y, y, 0.5), y);
+    y = su2_fma(y, su2_fma(-h * y, y, 0.5), y);
+    y = su2_fma(y, su2_fma(-h * y, y, 0.5), y);
+    return x == 0.0 ? __builtin_inf() : y;
+}
 
 /* sqrtf (functions/mc_updates.h:56), correctly rounded. */
-SU2_FM_FN float su2_sqrtf(float x) { return (float)__builtin_sqrt((double)x); }
+SU2_FM_FN float su2_sqrtf(float x) { return __builtin_sqrtf(x); }
 
 SU2_FM_FN float su2_fmaxf(float a, float b)
 {
@@ -136,10 +169,56 @@ SU2_FM_FN double su2_log_d(double x)
     return su2_log_normal_d(x);
 }
 
-/* logf of the reference (functions/mc_updates.h:49) for finite x > 0.
- * Every positive float, subnormals included, is a normal double, so no
- * range scaling is needed; the heat-bath calls it on draws in (0, 1]. */
-SU2_FM_FN float su2_logf(float x) { return (float)su2_log_normal_d((double)x); }
+SU2_FM_FN uint32_t su2_fm_fbits(float x)
+{
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    return v.u;
+}
+
+SU2_FM_FN float su2_fm_ffrom_bits(uint32_t u)
+{
+    union { float f; uint32_t u; } v;
+    v.u = u;
+    return v.f;
+}
+
+/* Chebyshev coefficients (floats) of U(f) = (log1p(f) / f - 1 + f / 2) / f^2
+ * on the mantissa range below (gen_fmath_coeffs.py, degree 8). */
+#define SU2_FM_LOGU_C0 0x1.5555540000000p-2f
+#define SU2_FM_LOGU_C1 -0x1.fffffc0000000p-3f
+#define SU2_FM_LOGU_C2 0x1.999d5a0000000p-3f
+#define SU2_FM_LOGU_C3 -0x1.555b4a0000000p-3f
+#define SU2_FM_LOGU_C4 0x1.23d21a0000000p-3f
+#define SU2_FM_LOGU_C5 -0x1.fcf4c60000000p-4f
+#define SU2_FM_LOGU_C6 0x1.dea2800000000p-4f
+#define SU2_FM_LOGU_C7 -0x1.d635c00000000p-4f
+#define SU2_FM_LOGU_C8 0x1.1d8eb00000000p-4f
+#define SU2_LN2 0x1.62e42fefa39efp-1
+
+/* logf of the reference (functions/mc_updates.h:49) for positive normal
+ * floats (the heat-bath calls it on draws in [2^-32, 1]).  x = 2^e m with
+ * m in [0x3f3504f3, 0x3fb504f2] by integer operations on the float's bits:
+ * subtracting the bits of 0.70710677 leaves e (signed) above the mantissa
+ * field, and adding them back to the mantissa field alone gives m. */
+SU2_FM_FN float su2_logf(float x)
+{
+    const uint32_t ix = su2_fm_fbits(x) - 0x3f3504f3u;
+    const int e = (int)(ix >> 23) - (int)((ix >> 31) << 9); /* arithmetic ix >> 23 */
+    const float ff = su2_fm_ffrom_bits((ix & 0x007fffffu) + 0x3f3504f3u) - 1.0f; /* exact */
+    float u = SU2_FM_LOGU_C8;
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C7);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C6);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C5);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C4);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C3);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C2);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C1);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C0);
+    const double f = (double)ff;
+    const double p = su2_fma(f, su2_fma(f, (double)u, -0.5), 1.0); /* log1p(f) / f */
+    return (float)su2_fma((double)e, SU2_LN2, f * p);
+}
 
 /* Exponential in double. */
 SU2_FM_FN double su2_exp_d(double x)
@@ -218,16 +297,45 @@ SU2_FM_FN void su2_sincospi_d(double a, double *sp, double *cp)
     *cp = su2_fm_sinpi_k(0.5 - q);
 }
 
-/* cospif of the reference (functions/mc_updates.h:48). */
-SU2_FM_FN float su2_cospif(float x) { return (float)su2_cospi_d((double)x); }
+/* Chebyshev coefficients of sin(pi d) / d as a polynomial in d^2, |d| <= 1/2
+ * (gen_fmath_coeffs.py, degree 5). */
+#define SU2_FM_SIN_C0 0x1.921fb54425516p+1
+#define SU2_FM_SIN_C1 -0x1.4abbce51c31dap+2
+#define SU2_FM_SIN_C2 0x1.466bc06900900p+1
+#define SU2_FM_SIN_C3 -0x1.32d12ebe6b560p-1
+#define SU2_FM_SIN_C4 0x1.5014072bd9ca2p-4
+#define SU2_FM_SIN_C5 -0x1.cc9b5f3456b49p-8
 
-/* sincospif of the reference (functions/mc_updates.h:55). */
+/* sin(pi d) for |d| <= 1/2, accurate enough for a float result. */
+SU2_FM_FN double su2_fm_sinpi_f(double d)
+{
+    const double w = d * d;
+    double p = SU2_FM_SIN_C5;
+    p = su2_fma(w, p, SU2_FM_SIN_C4);
+    p = su2_fma(w, p, SU2_FM_SIN_C3);
+    p = su2_fma(w, p, SU2_FM_SIN_C2);
+    p = su2_fma(w, p, SU2_FM_SIN_C1);
+    p = su2_fma(w, p, SU2_FM_SIN_C0);
+    return d * p;
+}
+
+/* cospif of the reference (functions/mc_updates.h:48) for finite x:
+ * cos(pi q) = sin(pi (1/2 - q)), q = |x - 2 round(x/2)|, all exact. */
+SU2_FM_FN float su2_cospif(float x)
+{
+    int neg;
+    const double q = su2_fm_reduce2((double)x, &neg);
+    return (float)su2_fm_sinpi_f(0.5 - q);
+}
+
+/* sincospif of the reference (functions/mc_updates.h:55) for finite x. */
 SU2_FM_FN void su2_sincospif(float x, float *sp, float *cp)
 {
-    double s, c;
-    su2_sincospi_d((double)x, &s, &c);
-    *sp = (float)s;
-    *cp = (float)c;
+    int neg;
+    const double q = su2_fm_reduce2((double)x, &neg);
+    const double s = su2_fm_sinpi_f(q < 1.0 - q ? q : 1.0 - q);
+    *sp = (float)(neg ? -s : s);
+    *cp = (float)su2_fm_sinpi_f(0.5 - q);
 }
 
 #endif /* SU2_FMATH_H */

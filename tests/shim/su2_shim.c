@@ -4,6 +4,7 @@
  * elementary functions, the MRG32k3a generator and the single-link update
  * bodies.  Test infrastructure only.
  */
+#include <math.h>
 #include <string.h>
 
 #include "su2_fmath.h"
@@ -71,4 +72,114 @@ double shim_q_scalar(const double *A, const double *B, int kind)
     case 1: return su2_q_retr(a, b);
     default: return su2_q_mul_nd_re(a, b);
     }
+}
+
+/* ---- exhaustive accuracy sweeps of the float elementary functions ----
+ *
+ * Error of su2_logf (which = 0), su2_cospif (1), and the sine (2) and
+ * cosine (3) of su2_sincospif over every float whose bit pattern lies in
+ * [lo, hi), in ulps of the exact result: |got - ref| / ulp(ref), ulp(y) =
+ * 2^(floor(log2 |y|) - 23) (subnormal spacing below 2^-126).  ref is libm's
+ * double-precision value, about 1e-16 relative; the trigonometric ones
+ * after an exact reduction, sin(pi d) with |d| <= 1/2, so that ref keeps its
+ * relative accuracy next to the zeros.  Returns the largest error; *worst
+ * = the bit pattern where it occurs; *not_cr = the number of results that
+ * differ from the correctly rounded float of ref.  `planted` (tests only)
+ * evaluates a deliberately wrong variant instead of the product function:
+ * 1 = su2_logf with its U polynomial's f^2 coefficient off by 1 %, 2 =
+ * cospif with the reduction slip cos(pi q) = sin(pi (1 - q)). */
+#define SHIM_PI 3.14159265358979323846
+
+static double ulp_of(double y)
+{
+    y = fabs(y);
+    if (y < 0x1p-126) return 0x1p-149;
+    int e;
+    frexp(y, &e); /* y = m 2^e, m in [0.5, 1) */
+    return ldexp(1.0, e - 1 - 23);
+}
+
+static float planted_logf(float x)
+{
+    const uint32_t ix = su2_fm_fbits(x) - 0x3f3504f3u;
+    const int e = (int)(ix >> 23) - (int)((ix >> 31) << 9);
+    const float ff = su2_fm_ffrom_bits((ix & 0x007fffffu) + 0x3f3504f3u) - 1.0f;
+    float u = SU2_FM_LOGU_C8;
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C7);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C6);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C5);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C4);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C3);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C2 * 1.01f);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C1);
+    u = su2_fmaf(ff, u, SU2_FM_LOGU_C0);
+    const double f = (double)ff;
+    const double p = su2_fma(f, su2_fma(f, (double)u, -0.5), 1.0);
+    return (float)su2_fma((double)e, SU2_LN2, f * p);
+}
+
+static float planted_cospif(float x)
+{
+    int neg;
+    const double q = su2_fm_reduce2((double)x, &neg);
+    return (float)su2_fm_sinpi_f(1.0 - q);
+}
+
+static double ref_value(int which, float x)
+{
+    if (which == 0) return log((double)x);
+    const double a = (double)x;
+    const double r = a - 2.0 * nearbyint(0.5 * a); /* exact, in [-1, 1] */
+    const double q = fabs(r);
+    if (which == 2) {
+        const double d = q < 1.0 - q ? q : 1.0 - q;
+        const double s = sin(SHIM_PI * d);
+        return r < 0 ? -s : s;
+    }
+    return sin(SHIM_PI * (0.5 - q));
+}
+
+double shim_fmath_max_ulp(int which, uint32_t lo, uint32_t hi, int planted, uint32_t *worst, long long *not_cr)
+{
+    double best = 0.0;
+    uint32_t wb = lo;
+    long long ncr = 0;
+#pragma omp parallel
+    {
+        double lbest = 0.0;
+        uint32_t lwb = lo;
+        long long lncr = 0;
+#pragma omp for schedule(static)
+        for (long long b = lo; b < (long long)hi; ++b) {
+            const float x = su2_fm_ffrom_bits((uint32_t)b);
+            float got;
+            if (which == 0) {
+                got = planted == 1 ? planted_logf(x) : su2_logf(x);
+            } else if (which == 1) {
+                got = planted == 2 ? planted_cospif(x) : su2_cospif(x);
+            } else {
+                float s, c;
+                su2_sincospif(x, &s, &c);
+                got = which == 2 ? s : c;
+            }
+            const double ref = ref_value(which, x);
+            const double err = fabs((double)got - ref) / ulp_of(ref);
+            if (got != (float)ref) ++lncr;
+            if (err > lbest || err != err) {
+                lbest = err != err ? 1e300 : err;
+                lwb = (uint32_t)b;
+            }
+        }
+#pragma omp critical
+        {
+            if (lbest > best) {
+                best = lbest;
+                wb = lwb;
+            }
+            ncr += lncr;
+        }
+    }
+    if (worst) *worst = wb;
+    if (not_cr) *not_cr = ncr;
+    return best;
 }

@@ -41,10 +41,26 @@ def _fma(a, b, c):
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
+def _rsqrt(x):
+    """The kernels' reciprocal square root: four Newton steps
+    y <- y + y (1/2 - (x/2) y^2) from the bit-level seed, each fused
+    multiply-add rounded once (its one-ulp accuracy is checked against mpmath
+    in test_fmath.py)."""
+    import struct
+
+    bits = struct.unpack("<Q", struct.pack("<d", x))[0]
+    y = struct.unpack("<d", struct.pack("<Q", 0x5FE6EB50C7B537A9 - (bits >> 1)))[0]
+    h = 0.5 * x
+    for _ in range(4):
+        y = _fma(y, _fma(-h * y, y, 0.5), y)
+    return y
+
+
 def _hot_link(r):
     a = [r.uniform_double() - 0.5 for _ in range(4)]
     det = _fma(a[3], a[3], _fma(a[2], a[2], _fma(a[1], a[1], a[0] * a[0])))
-    inv = 1.0 / math.sqrt(det)
+    inv = _rsqrt(det)
+    assert abs(inv * math.sqrt(det) - 1.0) < 3e-16
     return np.array([x * inv for x in a])
 
 
