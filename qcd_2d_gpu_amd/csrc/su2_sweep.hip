@@ -34,12 +34,17 @@ namespace {
  * all its t-planes for one strip of x-rows before the next, so the
  * t-neighbour rows of a strip are re-read from L2 a few hundred KB later
  * instead of two full planes (2.6 MB at 128^3) later.  Measured with FP64
- * links at 128^3: over-relaxation launch -10 %, heat-bath -5 %; FP32 links
- * (half the footprint) and planes of more than 64 blocks (256^3,
- * DRAM-bound) gain nothing from it and keep the slab order.  Strips are
- * 16 blocks wide where the plane's block count allows it, else 8: at 128^3
- * (32 blocks per plane) 16 against 8 gave heat-bath -4 %, step -2 %; 4 and
- * 32 were slower than 8.
+ * links at 128^3: over-relaxation launch -10 %, heat-bath -5 %.  At 256^3
+ * (128 blocks per plane) the L2-miss bytes of a heat-bath launch drop from
+ * 1.55 to 1.10 times the compulsory bytes, the t-neighbour planes no longer
+ * re-fetched through the Infinity Cache, but the launch gains only 1.7 %:
+ * there the time is set by the DRAM access pattern itself (a copy of the
+ * kernel with its arithmetic removed takes 99 % of its time).  FP32 links
+ * (half the footprint) gain nothing from it and keep the slab order.
+ * Strips are 16 blocks wide where the plane's block count allows it, else
+ * 8: at 128^3 (32 blocks per plane) 16 against 8 gave heat-bath -4 %, step
+ * -2 %; 4 and 32 were slower than 8, and at 256^3 8 was 5 % slower than
+ * 16.
  *
  * ORDER_COLUMNS (thin lattices, Lt <= 32, with bpp a multiple of 8): a slab
  * of Lt/8 <= 4 planes would send half or more of an XCD's t-neighbour reads
@@ -130,7 +135,7 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
     const bool whole = g.plane % (uint32_t)block == 0 && nb * (uint32_t)block == g.Vh;
     if (f64 && whole && g.Lt <= 32u && bpp % 8u == 0) {
         order = ORDER_COLUMNS;
-    } else if (f64 && whole && bpp % w == 0 && bpp <= 64 && (g.Vh / (uint32_t)block) % (8u * bpp) == 0) {
+    } else if (f64 && whole && bpp % w == 0 && bpp <= 128 && (g.Vh / (uint32_t)block) % (8u * bpp) == 0) {
         order = ORDER_STRIPS;
         strip = w;
     }
