@@ -169,6 +169,20 @@ int su2_normalize(const su2_lattice *l, void *phi, int precision, void *stream);
 int su2_en_av(const su2_lattice *l, const void *phi, int precision, int pmc, double *res, double *res2, void *work,
               void *stream);
 
+/* No counterpart in the reference: su2_en_av for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels and summation
+ * order as su2_en_av, so the results are identical bit for bit, with the
+ * block partial sums in the library's own static device array (4 MiB per
+ * device, held until the process ends) instead of a work buffer; no
+ * allocation of any kind.  Lattices of at most SU2_EN_AV_NOALLOC_MAX_SITES
+ * sites, else SU2_EINVAL.  All calls on one device share that array: calls
+ * that overlap on one device (several streams or host threads) race and
+ * must be serialised by the caller. */
+#define SU2_EN_AV_NOALLOC_MAX_SITES (1ll << 26)
+int su2_en_av_noalloc(const su2_lattice *l, const void *phi, int precision, int pmc, double *res, double *res2,
+                      void *stream);
+
 /* cudaMemset(accept, 0, V * sizeof(int)) of kernel.cu:145,190, done with a
  * fill kernel. */
 int su2_accept_clear(const su2_lattice *l, int *accept, void *stream);

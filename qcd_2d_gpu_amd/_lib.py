@@ -98,6 +98,7 @@ PROTOTYPES = {
     "su2_mc_update": (_I, [_L, _P, _I, _P, _U32P, _I, _I, _I, _I, _I, _P]),
     "su2_normalize": (_I, [_L, _P, _I, _P]),
     "su2_en_av": (_I, [_L, _P, _I, _I, _P, _P, _P, _P]),
+    "su2_en_av_noalloc": (_I, [_L, _P, _I, _I, _P, _P, _P]),
     "su2_accept_clear": (_I, [_L, _P, _P]),
     "su2_mean_lattice": (_I, [_L, _P, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double), _P]),
     "su2_field_to_ref": (_I, [_L, _P, _I, _P, _P]),

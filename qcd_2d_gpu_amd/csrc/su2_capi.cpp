@@ -299,6 +299,20 @@ int su2_en_av(const su2_lattice *l, const void *phi, int precision, int pmc, dou
     return e == hipSuccess ? SU2_OK : hip_fail(e, "en_av launch");
 }
 
+int su2_en_av_noalloc(const su2_lattice *l, const void *phi, int precision, int pmc, double *res, double *res2,
+                      void *stream)
+{
+    int rc = check_lattice(l);
+    if (rc) return rc;
+    if ((rc = check_prec(precision))) return rc;
+    if (!phi || !res || !res2) return fail(SU2_EINVAL, "NULL pointer passed to en_av_noalloc");
+    if ((long long)l->V > SU2_EN_AV_NOALLOC_MAX_SITES)
+        return fail(SU2_EINVAL, "en_av_noalloc: %d sites exceed its scratch limit of %lld sites (use su2_en_av)", l->V,
+                    (long long)SU2_EN_AV_NOALLOC_MAX_SITES);
+    hipError_t e = su2_launch_en_av_noalloc(pmc ? 1 : 0, precision, phi, geom(l), res, res2, S(stream));
+    return e == hipSuccess ? SU2_OK : hip_fail(e, "en_av_noalloc launch");
+}
+
 int su2_accept_clear(const su2_lattice *l, int *accept, void *stream)
 {
     int rc = check_lattice(l);

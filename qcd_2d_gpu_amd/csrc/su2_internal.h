@@ -147,6 +147,12 @@ hipError_t su2_launch_normalize(int f32, void *U, const su2_kgeom &g, hipStream_
 hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g, double *res, double *res2,
                             double *work, hipStream_t st);
 size_t su2_en_av_work_doubles(const su2_kgeom &g);
+/* en_av with its partial sums in the library's own static device array (no
+ * allocation); lattices of at most SU2_EN_AV_NOALLOC_MAX_SITES sites (=
+ * SU2_EN_AV_NOALLOC_MAX_SITES of su2mc.h). */
+#define SU2_EN_AV_NOALLOC_MAX_SITES (1ll << 26)
+hipError_t su2_launch_en_av_noalloc(int pmc, int f32, const void *U, const su2_kgeom &g, double *res, double *res2,
+                                    hipStream_t st);
 hipError_t su2_launch_to_ref(int f32, const void *U, double *phi_ref, const su2_kgeom &g, hipStream_t st);
 hipError_t su2_launch_from_ref(int f32, const double *phi_ref, void *U, const su2_kgeom &g, hipStream_t st);
 hipError_t su2_launch_fill_i32(int *p, int value, size_t n, hipStream_t st);

@@ -228,6 +228,38 @@ __global__ __launch_bounds__(256) void k_stream_probe(su2_v4u *__restrict__ dst,
     }
 }
 
+/* Partial sums of su2_en_av_noalloc: two doubles per pass-1 block of a
+ * lattice of at most SU2_EN_AV_NOALLOC_MAX_SITES sites (2^25 half-sites,
+ * 2^17 chunks of 256, two blocks each): 4 MiB, held by each device the
+ * library runs on until the process ends.  Every call on a device writes
+ * every partial it reads, so nothing in it needs clearing; calls that
+ * overlap on one device (other streams, other host threads) race on it. */
+#define SU2_EN_AV_SCRATCH_DOUBLES ((size_t)2 * 2 * (size_t)(SU2_EN_AV_NOALLOC_MAX_SITES / 2 / SU2_EN_BLOCK))
+static __device__ __align__(256) double scratch[SU2_EN_AV_SCRATCH_DOUBLES];
+
+/* The device address of `scratch`, looked up once per device with that
+ * device current (host code never passes the array's name to a kernel: in
+ * host code it is a stand-in, not a device address). */
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[64];
+
+hipError_t scratch_address(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 /* The jump tables go to each device's __constant__ memory once; set-up may
  * be called from several host threads (a server thread and its caller), so
  * the check and the upload are serialised. */
@@ -332,6 +364,16 @@ hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g,
     const double invV = 1.0 / (2.0 * (double)g.Vh);
     hipLaunchKernelGGL(k_en_av_final, dim3(1), dim3(SU2_EN_FINAL_BLOCK), 0, st, (const double *)work, nb, invV, res, res2);
     return hipGetLastError();
+}
+
+hipError_t su2_launch_en_av_noalloc(int pmc, int f32, const void *U, const su2_kgeom &g, double *res, double *res2,
+                                    hipStream_t st)
+{
+    if (su2_en_av_work_doubles(g) > SU2_EN_AV_SCRATCH_DOUBLES) return hipErrorInvalidValue;
+    double *work = nullptr;
+    hipError_t e = scratch_address(&work);
+    if (e != hipSuccess) return e;
+    return su2_launch_en_av(pmc, f32, U, g, res, res2, work, st);
 }
 
 hipError_t su2_launch_to_ref(int f32, const void *U, double *phi_ref, const su2_kgeom &g, hipStream_t st)
