@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define SU2MC_ABI_VERSION 2
+#define SU2MC_ABI_VERSION 3
 
 /* ---- status codes ---- */
 #define SU2_OK 0
@@ -130,10 +130,12 @@ int su2_haar_init(const su2_lattice *l, void *phi, int precision, uint32_t *stat
  * six reference kernels leave it after the sweep (the heat-bath and
  * over-relaxation, which accept every update, add a site's count for the
  * whole sweep in its t launch; in between, the counters are not defined).
- * Couplings (from *l): the heat-bath needs a finite beta > 0 (its
- * Kennedy-Pendleton loop accepts with probability of order beta |A| for
- * small beta, so runs below beta ~ 0.1 get slow, and never at beta <= 0);
- * Metropolis needs 0 <= epsilon <= 1; otherwise SU2_EINVAL. */
+ * Couplings (from *l): the heat-bath needs a finite beta >=
+ * SU2_HB_BETA_MIN (its Kennedy-Pendleton trial accepts with probability
+ * about 1.26 (beta |A|)^1.5 for small beta |A|: some 800 trials per update at
+ * 0.01, never at beta <= 0); Metropolis needs 0 <= epsilon <= 1; otherwise
+ * SU2_EINVAL. */
+#define SU2_HB_BETA_MIN 0.01f
 int su2_mc_up_per(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int algo,
                   int update, void *stream);
 
@@ -328,6 +330,8 @@ typedef struct su2_job {
     int n_cor;           /* sweeps per measurement (>= 1) */
     int or_sweeps;       /* over-relaxation sweeps after each update sweep (<= engine max_or) */
     int normalize_every; /* reunitarisation cadence (0: 100, the reference's) */
+    int fp64_twin;       /* server of SU2_PREC_F32 only (engines ignore it): also run the job with FP64 links and
+                            report the plaquette difference in its su2_job_result */
 } su2_job;
 
 typedef struct su2_engine_config {
@@ -378,7 +382,11 @@ int su2_engine_memory(su2_engine *e, size_t *used, size_t *capacity);
  * admitted as soon as a slot and arena memory are free (first that fits,
  * in arrival order), run in the shared ticks with everything else resident,
  * and are retired with their results when done.  Times are seconds on the
- * server's steady clock.
+ * server's steady clock.  On an FP32-link server a job with fp64_twin also
+ * runs, tick for tick, in a second engine with FP64 links that the server
+ * thread creates on the first such job (an arena of the same budget): it is
+ * admitted when both halves fit, and its result states the plaquette
+ * difference.
  */
 typedef struct su2_server su2_server;
 
@@ -388,6 +396,10 @@ typedef struct su2_job_result {
     double t_submit, t_admit, t_done;
     double E_mean, E_err, P_mean, P_err, accept;
     long long link_updates;
+    /* jobs with fp64_twin on an FP32-link server, else 0: the FP64 twin's
+     * average plaquette and the errors from min(20, n_meas) bins; P_diff =
+     * P_mean - P_fp64_twin, P_diff_sigma = the combined binned error */
+    double P_fp64_twin, P_fp64_twin_err, P_err_binned, P_diff, P_diff_sigma;
 } su2_job_result;
 
 int su2_server_create(const su2_engine_config *cfg, su2_server **out);
@@ -440,6 +452,9 @@ typedef struct su2_run_config {
     int device;         /* HIP device ordinal */
     int N_block;        /* progress lines during the measurement (reference 20) */
     int Nblock_term;    /* progress lines during the thermalisation (reference 10) */
+    int fp64_twin;      /* extension: with precision SU2_PREC_F32, also run the same configuration with FP64
+                           links (same seed, start and schedule, writing nothing) and state the plaquette
+                           difference in su2_run_result and the Energy_av_ file */
 } su2_run_config;
 
 typedef struct su2_run_result {
@@ -453,7 +468,24 @@ typedef struct su2_run_result {
     double seconds_meas; /* wall time of the measurement loop */
     long long link_updates; /* link updates performed (update + over-relaxation) */
     int Nterms;          /* total thermalisation sweeps of the configuration */
+    /* with fp64_twin (FP32 links), else 0: the FP64 twin's average plaquette
+     * and the errors from 20 bins of the measurements (fewer measurements:
+     * one per bin); P_diff_fp32_fp64 = P_mean - P_fp64_twin, P_diff_sigma =
+     * the combined binned error (twin runs are the twin's own link updates and
+     * are not counted in link_updates or the timings) */
+    double P_fp64_twin, P_fp64_twin_err, P_err_binned;
+    double P_diff_fp32_fp64, P_diff_sigma;
 } su2_run_result;
+
+/* Mean and standard error of a series from nbins equal blocks (trailing
+ * values that do not fill a block are dropped; nbins >= 2, n >= nbins):
+ * the binning README.md:74 of the reference recommends for final errors. */
+int su2_binned_mean_error(const double *m, int n, int nbins, double *mean, double *err);
+/* Two series (FP32 links against FP64, or any pair): the difference of
+ * their binned means and its combined binned error sqrt(e_a^2 + e_b^2);
+ * NULL skips an output. */
+int su2_series_compare(const double *a, const double *b, int n, int nbins, double *diff, double *sigma,
+                       double *diff_over_sigma);
 
 void su2_run_config_default(su2_run_config *cfg);
 int su2_run(const su2_run_config *cfg, su2_run_result *res);

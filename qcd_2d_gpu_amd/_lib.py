@@ -62,6 +62,7 @@ class Su2RunConfig(C.Structure):
         ("seed", C.c_uint64),
         ("outdir", C.c_char_p), ("nsave", C.c_char_p), ("nconfr", C.c_char_p), ("nconfs", C.c_char_p),
         ("verbose", C.c_int), ("device", C.c_int), ("N_block", C.c_int), ("Nblock_term", C.c_int),
+        ("fp64_twin", C.c_int),
     ]
 
 
@@ -72,6 +73,8 @@ class Su2RunResult(C.Structure):
         ("accept", C.c_double), ("det_mean", C.c_double),
         ("seconds_therm", C.c_double), ("seconds_meas", C.c_double),
         ("link_updates", C.c_longlong), ("Nterms", C.c_int),
+        ("P_fp64_twin", C.c_double), ("P_fp64_twin_err", C.c_double), ("P_err_binned", C.c_double),
+        ("P_diff_fp32_fp64", C.c_double), ("P_diff_sigma", C.c_double),
     ]
 
 
@@ -144,6 +147,9 @@ PROTOTYPES = {
     "su2_rng_upload": (_I, [_P, _P]),
     "su2_run_config_default": (None, [C.POINTER(Su2RunConfig)]),
     "su2_run": (_I, [C.POINTER(Su2RunConfig), C.POINTER(Su2RunResult)]),
+    "su2_binned_mean_error": (_I, [C.POINTER(C.c_double), _I, _I, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "su2_series_compare": (_I, [C.POINTER(C.c_double), C.POINTER(C.c_double), _I, _I, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -169,7 +175,7 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.su2_abi_version() != 2:
+    if lib.su2_abi_version() != 3:
         raise RuntimeError("libsu2mc ABI version mismatch")
     _lib = lib
     return lib

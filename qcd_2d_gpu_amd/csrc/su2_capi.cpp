@@ -66,16 +66,16 @@ su2_kgeom geom(const su2_lattice *l) { return su2_make_kgeom(l->t, l->x, l->y); 
 hipStream_t S(void *s) { return (hipStream_t)s; }
 
 /* Coupling checks of a link update (over-relaxation uses neither).  The
- * heat-bath's Kennedy-Pendleton loop accepts with probability ~ beta * |A|
+ * heat-bath's Kennedy-Pendleton loop accepts with probability ~ (beta |A|)^1.5
  * for small beta and never at beta <= 0 (its scale 0.5 / (|A| beta) is
- * infinite or negative), so beta must be positive; a Metropolis proposal
- * (sqrt(1 - eps^2), eps r) needs 0 <= eps <= 1. */
+ * infinite or negative), so beta must be at least SU2_HB_BETA_MIN; a
+ * Metropolis proposal (sqrt(1 - eps^2), eps r) needs 0 <= eps <= 1. */
 int check_couplings(const su2_lattice *l, int algo, int update)
 {
     if (update != SU2_UPDATE_LINK) return SU2_OK;
     if (!std::isfinite(l->beta)) return fail(SU2_EINVAL, "beta is not finite");
-    if (algo == SU2_ALGO_HEATBATH && !(l->beta > 0.0f))
-        return fail(SU2_EINVAL, "heat-bath needs beta > 0 (got %g)", (double)l->beta);
+    if (algo == SU2_ALGO_HEATBATH && !(l->beta >= SU2_HB_BETA_MIN))
+        return fail(SU2_EINVAL, "heat-bath needs beta >= %g (got %g)", (double)SU2_HB_BETA_MIN, (double)l->beta);
     if (algo == SU2_ALGO_METROPOLIS) {
         if (!(l->epsilon >= 0.0f && l->epsilon <= 1.0f))
             return fail(SU2_EINVAL, "Metropolis needs 0 <= epsilon <= 1 (got %g)", (double)l->epsilon);

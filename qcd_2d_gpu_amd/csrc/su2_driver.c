@@ -52,6 +52,47 @@ static void mean_variance(const double *m, int n, double *res)
     res[1] = s2 / n - res[0] * res[0];
 }
 
+/* Mean and standard error from nbins equal blocks of the series (trailing
+ * values that do not fill a block are dropped): the binning the reference's
+ * README recommends for final errors (README.md:74). */
+int su2_binned_mean_error(const double *m, int n, int nbins, double *mean, double *err)
+{
+    if (!m || !mean || !err || nbins < 2 || n < nbins)
+        return su2_set_error(SU2_EINVAL, "binned error needs nbins >= 2 and at least nbins values");
+    const int b = n / nbins;
+    double s = 0.0;
+    for (int k = 0; k < nbins; ++k) {
+        double bm = 0.0;
+        for (int i = 0; i < b; ++i) bm += m[k * b + i];
+        s += bm / b;
+    }
+    const double mu = s / nbins;
+    double s2 = 0.0; /* second pass: squared deviations of the bin means */
+    for (int k = 0; k < nbins; ++k) {
+        double bm = 0.0;
+        for (int i = 0; i < b; ++i) bm += m[k * b + i];
+        const double d = bm / b - mu;
+        s2 += d * d;
+    }
+    *mean = mu;
+    *err = sqrt(s2 / (nbins - 1) / nbins); /* unbiased variance of the bin means / nbins */
+    return SU2_OK;
+}
+
+int su2_series_compare(const double *a, const double *b, int n, int nbins, double *diff, double *sigma,
+                       double *diff_over_sigma)
+{
+    double ma, ea, mb, eb;
+    int rc = su2_binned_mean_error(a, n, nbins, &ma, &ea);
+    if (rc) return rc;
+    if ((rc = su2_binned_mean_error(b, n, nbins, &mb, &eb))) return rc;
+    const double sg = sqrt(ea * ea + eb * eb);
+    if (diff) *diff = ma - mb;
+    if (sigma) *sigma = sg;
+    if (diff_over_sigma) *diff_over_sigma = sg > 0.0 ? (ma - mb) / sg : 0.0;
+    return SU2_OK;
+}
+
 /* autocorrelation (functions/statistics.h:41-51), normalised by the
  * variance; lags with no pairs (i >= n) are reported as 0. */
 static void autocorrelation(const double *m, int n, int W, double mean, double var, double *out)
@@ -97,8 +138,15 @@ static int write_energy(const su2_run_config *c, double mean, double var, int N_
     return SU2_OK;
 }
 
+/* The FP32-link run against its FP64 twin (same seed, start and schedule):
+ * the average plaquettes with 20-bin errors and their difference in units of
+ * the combined error. */
+typedef struct twin_cmp {
+    double P64, P64_err, P32_err, diff, sigma;
+} twin_cmp;
+
 static int write_energy_av(const su2_run_config *c, double m1, double v1, double m2, double v2, int N_term, int N,
-                           int N_cor, double acc, const su2_lattice *l, const char *bound)
+                           int N_cor, double acc, const su2_lattice *l, const char *bound, const twin_cmp *tw)
 {
     /* save_energy_av (output.h:30-44) */
     char f[PATH_LEN];
@@ -113,6 +161,9 @@ static int write_energy_av(const su2_run_config *c, double m1, double v1, double
     fprintf(fp, "M Energy/Avplaquette Error\n");
     fprintf(fp, "%f %.10lf %.10lf\n", (double)l->beta, m1, sqrt(v1 / (N - 1)));
     fprintf(fp, "%f %.10lf %.10lf\n", (double)l->beta, m2, sqrt(v2 / (N - 1)));
+    if (tw) /* extension: only in FP32-link runs with an FP64 twin */
+        fprintf(fp, "FP64_twin_plaquette %.10lf %.10lf FP32_minus_FP64 %.10lf sigma %.10lf diff_over_sigma %.4lf\n",
+                tw->P64, tw->P64_err, tw->diff, tw->sigma, tw->sigma > 0.0 ? tw->diff / tw->sigma : 0.0);
     fclose(fp);
     return SU2_OK;
 }
@@ -337,11 +388,32 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
     run_ctx r;
     memset(&r, 0, sizeof r);
     r.c = c;
-    double *h_phi = NULL, *en = NULL, *en2 = NULL, *autoc = NULL, *autoc2 = NULL;
+    double *h_phi = NULL, *en = NULL, *en2 = NULL, *autoc = NULL, *autoc2 = NULL, *p64 = NULL;
+    const int twin = c->precision == SU2_PREC_F32 && c->fp64_twin;
+    twin_cmp tw;
+    memset(&tw, 0, sizeof tw);
     uint32_t *hst = NULL;
     int rc = su2_lattice_init(&r.l, c->Lt, c->Lx, c->Ly, c->beta, c->epsilon, c->upd);
     if (rc) return rc;
     if ((rc = su2_set_device(c->device))) return rc;
+    if (twin) {
+        /* the FP64 twin: the same run with FP64 links, writing nothing */
+        su2_run_config c64 = *c;
+        c64.precision = SU2_PREC_F64;
+        c64.fp64_twin = 0;
+        c64.save = 1;
+        c64.sconf = 1;
+        c64.verbose = 0;
+        su2_run_result r64;
+        memset(&r64, 0, sizeof r64);
+        if (!(p64 = malloc((size_t)c->N * sizeof(double))))
+            return su2_set_error(SU2_ENOMEM, "host buffer of %zu bytes", (size_t)c->N * sizeof(double));
+        r64.plaquette = p64;
+        if ((rc = su2_run(&c64, &r64))) {
+            free(p64);
+            return rc;
+        }
+    }
     su2_lattice *l = &r.l;
     const int prec = r.prec = c->precision;
     const char *bound = bound_name(c->boundary);
@@ -458,7 +530,7 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
                 CK(acceptance(&r, i, &saccept));
                 CK(write_energy(c, energy[0], energy[1], Nterms, N, c->N_cor, saccept, l, bound));
                 CK(write_energy_av(c, energy[0], energy[1], energy2[0], energy2[1], Nterms, N, c->N_cor, saccept, l,
-                                   bound));
+                                   bound, NULL));
                 CK(write_autocorrelation(c, "Autocorrelation_", autoc, c->W, i, c->N_cor, Nterms, l, bound));
                 CK(write_autocorrelation(c, "Autocorrelation_av_", autoc2, c->W, i, c->N_cor, Nterms, l, bound));
                 CK(write_en_seq(c, en2, en, Nterms, i, c->N_cor, 1.0, l, bound));
@@ -475,9 +547,17 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
     autocorrelation(en, N, c->W, energy[0], energy[1], autoc);
     mean_variance(en2, N, energy2);
     autocorrelation(en2, N, c->W, energy2[0], energy2[1], autoc2);
+    if (twin) {
+        double z;
+        CK(su2_series_compare(en2, p64, N, N < 20 ? N : 20, &tw.diff, &tw.sigma, &z));
+        double m32;
+        CK(su2_binned_mean_error(en2, N, N < 20 ? N : 20, &m32, &tw.P32_err));
+        CK(su2_binned_mean_error(p64, N, N < 20 ? N : 20, &tw.P64, &tw.P64_err));
+    }
     if (c->save == 0) { /* kernel.cu:254-260 */
         CK(write_energy(c, energy[0], energy[1], Nterms, N, c->N_cor, saccept, l, bound));
-        CK(write_energy_av(c, energy[0], energy[1], energy2[0], energy2[1], Nterms, N, c->N_cor, saccept, l, bound));
+        CK(write_energy_av(c, energy[0], energy[1], energy2[0], energy2[1], Nterms, N, c->N_cor, saccept, l, bound,
+                           twin ? &tw : NULL));
         CK(write_en_seq(c, en2, en, Nterms, N, c->N_cor, 1.0, l, bound));
         CK(write_autocorrelation(c, "Autocorrelation_", autoc, c->W, N, c->N_cor, Nterms, l, bound));
         CK(write_autocorrelation(c, "Autocorrelation_av_", autoc2, c->W, N, c->N_cor, Nterms, l, bound));
@@ -502,9 +582,15 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
     res->seconds_meas = t2 - t1;
     res->link_updates = r.updates;
     res->Nterms = Nterms;
+    res->P_fp64_twin = twin ? tw.P64 : 0.0;
+    res->P_fp64_twin_err = twin ? tw.P64_err : 0.0;
+    res->P_err_binned = twin ? tw.P32_err : 0.0;
+    res->P_diff_fp32_fp64 = twin ? tw.diff : 0.0;
+    res->P_diff_sigma = twin ? tw.sigma : 0.0;
     rc = SU2_OK;
 
 done:
+    free(p64);
     free(h_phi);
     free(hst);
     free(en);

@@ -34,6 +34,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -92,11 +93,14 @@ int check_job(const su2_job *j, int max_or)
     if (j->algo != SU2_ALGO_HEATBATH && j->algo != SU2_ALGO_METROPOLIS) return su2_set_error(SU2_EINVAL, "job algo");
     if (j->algo == SU2_ALGO_METROPOLIS && j->hits < 1) return su2_set_error(SU2_EINVAL, "Metropolis hits < 1");
     if (!std::isfinite(j->beta)) return su2_set_error(SU2_EINVAL, "beta is not finite");
-    /* the Kennedy-Pendleton loop never accepts at beta <= 0 */
-    if (j->algo == SU2_ALGO_HEATBATH && !(j->beta > 0.0f)) return su2_set_error(SU2_EINVAL, "heat-bath needs beta > 0");
+    /* the Kennedy-Pendleton loop never accepts at beta <= 0 and crawls
+     * below SU2_HB_BETA_MIN */
+    if (j->algo == SU2_ALGO_HEATBATH && !(j->beta >= SU2_HB_BETA_MIN))
+        return su2_set_error(SU2_EINVAL, "heat-bath needs beta >= %g", (double)SU2_HB_BETA_MIN);
     if (j->algo == SU2_ALGO_METROPOLIS && !(j->epsilon >= 0.0f && j->epsilon <= 1.0f))
         return su2_set_error(SU2_EINVAL, "Metropolis needs 0 <= epsilon <= 1");
     if (j->start < SU2_START_HOT || j->start > SU2_START_HAAR) return su2_set_error(SU2_EINVAL, "job start");
+    if (j->fp64_twin != 0 && j->fp64_twin != 1) return su2_set_error(SU2_EINVAL, "job fp64_twin must be 0 or 1");
     return SU2_OK;
 }
 
@@ -655,6 +659,8 @@ int su2_engine_autotune(su2_engine *e, int *block, int *grid)
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
+            /* ticks enqueued before may still run their graphs */
+            ENG_HIP(hipStreamSynchronize(e->st));
             e->block = it->second.first;
             e->grid = it->second.second;
             e->dirty = true;
@@ -753,6 +759,7 @@ struct pending_job {
 struct running_job {
     long long id;
     int slot;
+    int slot64; /* the FP64 twin's slot in the server's FP64 engine, or -1 */
     double t_submit, t_admit;
     su2_job job;
 };
@@ -781,9 +788,19 @@ double mean_err(const std::vector<double> &m, double *err)
 
 struct su2_server {
     su2_engine *e = nullptr;
+    su2_engine *e64 = nullptr; /* FP64 twins of an FP32-link server; made on the first twin job */
+    hipEvent_t ev64[2] = {nullptr, nullptr};
+    su2_engine_config cfg{};
     std::thread th;
     std::mutex mu;
     std::condition_variable cv, idle_cv;
+    int created = 0; /* 0 pending, 1 engine ready, -1 creation failed; guarded by mu */
+    int create_rc = SU2_OK;
+    std::string create_msg;
+    /* the engine's limits, copied at creation: submit checks jobs with them
+     * and never touches the engine, which the server thread owns */
+    int max_or = 1, precision = SU2_PREC_F64;
+    size_t cap = 0;
     std::deque<pending_job> queue;        /* guarded by mu */
     std::map<long long, finished_job> finished; /* guarded by mu */
     std::vector<running_job> running;     /* server thread only */
@@ -816,6 +833,22 @@ struct su2_server {
         double acc = 0.0;
         int rc = engine_read_impl(e, r.slot, f.E.data(), f.P.data(), &acc, nullptr, nullptr, false);
         su2_engine_remove(e, r.slot);
+        if (r.slot64 >= 0) {
+            std::vector<double> P64((size_t)r.job.n_meas);
+            int rc64 = engine_read_impl(e64, r.slot64, nullptr, P64.data(), nullptr, nullptr, nullptr, false);
+            su2_engine_remove(e64, r.slot64);
+            const int n = r.job.n_meas, nb = n < 20 ? n : 20;
+            if (rc64 == SU2_OK && nb >= 2) {
+                double m32 = 0.0;
+                rc64 = su2_binned_mean_error(f.P.data(), n, nb, &m32, &f.res.P_err_binned);
+                if (rc64 == SU2_OK)
+                    rc64 = su2_binned_mean_error(P64.data(), n, nb, &f.res.P_fp64_twin, &f.res.P_fp64_twin_err);
+                if (rc64 == SU2_OK)
+                    rc64 = su2_series_compare(f.P.data(), P64.data(), n, nb, &f.res.P_diff, &f.res.P_diff_sigma,
+                                              nullptr);
+            }
+            if (rc == SU2_OK) rc = rc64;
+        }
         f.res.status = rc;
         f.res.E_mean = mean_err(f.E, &f.res.E_err);
         f.res.P_mean = mean_err(f.P, &f.res.P_err);
@@ -833,6 +866,78 @@ struct su2_server {
         if (error == SU2_OK) error = rc;
         stop = true;
         idle_cv.notify_all();
+    }
+
+    /* The server thread creates the engine (its arena, tables and stream),
+     * uses it and, after the loop, destroys it: every HIP object of the
+     * engine is made and used by this one thread. */
+    void run()
+    {
+        su2_engine *eng = nullptr;
+        const int rc = su2_engine_create(&cfg, &eng);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            e = eng;
+            create_rc = rc;
+            if (rc != SU2_OK) {
+                create_msg = su2_last_error();
+            } else {
+                max_or = eng->cfg.max_or;
+                precision = eng->cfg.precision;
+                cap = eng->cap;
+            }
+            created = rc == SU2_OK ? 1 : -1;
+        }
+        idle_cv.notify_all();
+        if (rc != SU2_OK) return;
+        loop();
+        if (e64) {
+            su2_engine_destroy(e64);
+            for (hipEvent_t &x : ev64)
+                if (x) hipEventDestroy(x);
+        }
+        su2_engine_destroy(e);
+    }
+
+    /* the FP64 engine of the twins, created on first use by this thread */
+    int twin_engine()
+    {
+        if (e64) return SU2_OK;
+        su2_engine_config c = cfg;
+        c.precision = SU2_PREC_F64;
+        su2_engine *x = nullptr;
+        int rc = su2_engine_create(&c, &x);
+        if (rc) return rc;
+        hipError_t he = hipEventCreateWithFlags(&ev64[0], hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&ev64[1], hipEventDisableTiming);
+        if (he != hipSuccess) {
+            for (hipEvent_t &v : ev64)
+                if (v) hipEventDestroy(v);
+            ev64[0] = ev64[1] = nullptr;
+            su2_engine_destroy(x);
+            return su2_set_error(SU2_EHIP, "twin events: %s", hipGetErrorString(he));
+        }
+        e64 = x;
+        return SU2_OK;
+    }
+
+    /* admits a job and, for an FP64 twin, its twin: both or neither */
+    int admit_job(const su2_job &job, int *slot, int *slot64)
+    {
+        *slot64 = -1;
+        const bool twin = job.fp64_twin && cfg.precision == SU2_PREC_F32;
+        if (twin) {
+            int rc = twin_engine();
+            if (rc) return rc;
+        }
+        int rc = su2_engine_add(e, &job, slot);
+        if (rc || !twin) return rc;
+        rc = su2_engine_add(e64, &job, slot64);
+        if (rc) {
+            su2_engine_remove(e, *slot);
+            *slot64 = -1;
+        }
+        return rc;
     }
 
     void loop()
@@ -855,10 +960,10 @@ struct su2_server {
                 std::unique_lock<std::mutex> lk(mu);
                 /* admit in arrival order, first that fits */
                 for (auto it = queue.begin(); it != queue.end();) {
-                    int slot = -1;
-                    int rc = su2_engine_add(e, &it->job, &slot);
+                    int slot = -1, slot64 = -1;
+                    int rc = admit_job(it->job, &slot, &slot64);
                     if (rc == SU2_OK) {
-                        running.push_back({it->id, slot, it->t_submit, now(), it->job});
+                        running.push_back({it->id, slot, slot64, it->t_submit, now(), it->job});
                         it = queue.erase(it);
                     } else if (rc == SU2_EBUDGET) {
                         ++it;
@@ -891,15 +996,20 @@ struct su2_server {
                 int rc = su2_engine_tick(e, 1);
                 if (rc == SU2_OK && (he = hipEventRecord(ev[cur], e->st)) != hipSuccess)
                     rc = su2_set_error(SU2_EHIP, "server tick event: %s", hipGetErrorString(he));
+                /* the FP64 twins advance in the same tick, on their own stream */
+                if (rc == SU2_OK && e64) rc = su2_engine_tick(e64, 1);
+                if (rc == SU2_OK && e64 && (he = hipEventRecord(ev64[cur], e64->st)) != hipSuccess)
+                    rc = su2_set_error(SU2_EHIP, "server twin tick event: %s", hipGetErrorString(he));
                 if (rc != SU2_OK) {
                     fail(rc);
                     break;
                 }
                 ticked = true;
                 for (auto it = running.begin(); it != running.end();) {
-                    int d = 0, t = 0;
+                    int d = 0, t = 0, d64 = 0, t64 = 0;
                     su2_engine_job_status(e, it->slot, &d, &t);
-                    if (d == t) {
+                    if (it->slot64 >= 0) su2_engine_job_status(e64, it->slot64, &d64, &t64);
+                    if (d == t && d64 == t64) {
                         done_now.push_back(*it);
                         it = running.erase(it);
                     } else {
@@ -909,7 +1019,8 @@ struct su2_server {
             }
             /* retire the jobs the previous tick finished while this one runs */
             if (prev_valid) {
-                if ((he = hipEventSynchronize(ev[cur ^ 1])) != hipSuccess) {
+                if ((he = hipEventSynchronize(ev[cur ^ 1])) != hipSuccess ||
+                    (e64 && (he = hipEventSynchronize(ev64[cur ^ 1])) != hipSuccess)) {
                     fail(su2_set_error(SU2_EHIP, "server tick wait: %s", hipGetErrorString(he)));
                     break;
                 }
@@ -924,7 +1035,8 @@ struct su2_server {
             }
         }
         if (prev_valid && error == SU2_OK) {
-            if ((he = hipEventSynchronize(ev[cur ^ 1])) == hipSuccess)
+            if ((he = hipEventSynchronize(ev[cur ^ 1])) == hipSuccess &&
+                (!e64 || (he = hipEventSynchronize(ev64[cur ^ 1])) == hipSuccess))
                 for (auto &r : prev_done) retire(r);
             else
                 fail(su2_set_error(SU2_EHIP, "server tick wait: %s", hipGetErrorString(he)));
@@ -941,14 +1053,22 @@ extern "C" {
 int su2_server_create(const su2_engine_config *cfg, su2_server **out)
 {
     if (!cfg || !out) return su2_set_error(SU2_EINVAL, "NULL");
-    su2_engine_config c = *cfg;
-    c.use_graph = cfg->use_graph;
-    su2_engine *e = nullptr;
-    int rc = su2_engine_create(&c, &e);
-    if (rc) return rc;
+    *out = nullptr;
     su2_server *s = new su2_server();
-    s->e = e;
-    s->th = std::thread([s] { s->loop(); });
+    s->cfg = *cfg;
+    s->th = std::thread([s] { s->run(); });
+    int rc;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->idle_cv.wait(lk, [s] { return s->created != 0; });
+        rc = s->create_rc;
+    }
+    if (rc != SU2_OK) {
+        s->th.join();
+        const std::string msg = s->create_msg; /* recorded on the server thread */
+        delete s;
+        return su2_set_error(rc, "%s", msg.c_str());
+    }
     *out = s;
     return SU2_OK;
 }
@@ -961,9 +1081,8 @@ int su2_server_destroy(su2_server *s)
         s->stop = true;
     }
     s->cv.notify_all();
-    if (s->th.joinable()) s->th.join();
+    if (s->th.joinable()) s->th.join(); /* the thread destroys its engine */
     int rc = s->error;
-    su2_engine_destroy(s->e);
     delete s;
     return rc;
 }
@@ -971,9 +1090,9 @@ int su2_server_destroy(su2_server *s)
 int su2_server_submit(su2_server *s, const su2_job *job, long long *id)
 {
     if (!s || !job || !id) return su2_set_error(SU2_EINVAL, "NULL");
-    int rc = check_job(job, s->e->cfg.max_or);
+    int rc = check_job(job, s->max_or);
     if (rc) return rc;
-    if (layout_of(*job, s->e->cfg.precision).total > s->e->cap)
+    if (layout_of(*job, s->precision).total > s->cap)
         return su2_set_error(SU2_EBUDGET, "job larger than the whole memory budget");
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return su2_set_error(SU2_ECANCELLED, "server is stopping");

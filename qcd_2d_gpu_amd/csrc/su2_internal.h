@@ -109,6 +109,14 @@ static inline su2_kgeom su2_make_kgeom(int Lt, int Lx, int Ly)
     return g;
 }
 
+/* Smallest heat-bath coupling accepted on the host (= SU2_HB_BETA_MIN of
+ * su2mc.h).  A Kennedy-Pendleton trial accepts with probability about
+ * 1.26 k^1.5 at small k = beta |A| (|A| <= 4, the staple norm): some 800
+ * trials per update at k = 0.01, 25 000 at 0.001, 10^9 at 1e-6, where a
+ * workgroup would loop for minutes.  beta >= 0.01 keeps k small only where
+ * the staple itself nearly vanishes. */
+#define SU2_HB_BETA_MIN 0.01f
+
 /* Physics parameters of one launch.  beta and eps arrive as the float values
  * of the reference's struct lattice (lattice_setup.h:10-11) widened to
  * double, as the reference's heat_bath(double beta) does. */

@@ -17,7 +17,7 @@ static void usage(void)
             "             [--readconf 0|1|2] [--init 0|1|2] [--sconf 0|1] [--save 0|1] [--N n] [--N_term n]\n"
             "             [--N_cor n] [--Nsave n] [--epsilon f] [--upd n] [--W n] [--or n] [--precision 0|1]\n"
             "             [--normalize_every n] [--seed n] [--outdir dir] [--nsave s] [--nconfr s]\n"
-            "             [--nconfs s] [--device n] [--N_block n] [--Nblock_term n] [--quiet]\n");
+            "             [--nconfs s] [--device n] [--N_block n] [--Nblock_term n] [--fp64_twin 0|1] [--quiet]\n");
 }
 
 int main(int argc, char **argv)
@@ -66,6 +66,7 @@ int main(int argc, char **argv)
         INT_FLAG("device", device)
         INT_FLAG("N_block", N_block)
         INT_FLAG("Nblock_term", Nblock_term)
+        INT_FLAG("fp64_twin", fp64_twin)
 #undef INT_FLAG
         if (!strcmp(k, "--beta")) c.beta = (float)atof(v);
         else if (!strcmp(k, "--epsilon")) c.epsilon = (float)atof(v);
@@ -89,5 +90,9 @@ int main(int argc, char **argv)
     }
     printf("Time therm %.3f s meas %.3f s, %.3e link updates/s\n", r.seconds_therm, r.seconds_meas,
            (double)r.link_updates / (r.seconds_therm + r.seconds_meas));
+    if (c.precision == SU2_PREC_F32 && c.fp64_twin)
+        printf("FP32 links: plaquette %.10f +- %.10f, FP64 twin %.10f +- %.10f, difference %.3f sigma\n", r.P_mean,
+               r.P_err_binned, r.P_fp64_twin, r.P_fp64_twin_err,
+               r.P_diff_sigma > 0 ? r.P_diff_fp32_fp64 / r.P_diff_sigma : 0.0);
     return 0;
 }

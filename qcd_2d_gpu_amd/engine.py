@@ -22,7 +22,7 @@ class Su2Job(C.Structure):
         ("beta", C.c_float), ("epsilon", C.c_float), ("hits", C.c_int),
         ("bc", C.c_int), ("algo", C.c_int), ("start", C.c_int), ("seed", C.c_uint64),
         ("n_therm", C.c_int), ("n_meas", C.c_int), ("n_cor", C.c_int), ("or_sweeps", C.c_int),
-        ("normalize_every", C.c_int),
+        ("normalize_every", C.c_int), ("fp64_twin", C.c_int),
     ]
 
 
@@ -39,6 +39,8 @@ class Su2JobResult(C.Structure):
         ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
         ("E_mean", C.c_double), ("E_err", C.c_double), ("P_mean", C.c_double), ("P_err", C.c_double),
         ("accept", C.c_double), ("link_updates", C.c_longlong),
+        ("P_fp64_twin", C.c_double), ("P_fp64_twin_err", C.c_double), ("P_err_binned", C.c_double),
+        ("P_diff", C.c_double), ("P_diff_sigma", C.c_double),
     ]
 
 
@@ -85,10 +87,12 @@ class Job:
     epsilon: float = 0.1
     hits: int = 1
     normalize_every: int = 100
+    fp64_twin: int = 0  # FP32-link server: also run with FP64 links and state the plaquette difference
 
     def c(self) -> Su2Job:
         return Su2Job(self.Lt, self.Lx, self.Ly, self.beta, self.epsilon, self.hits, self.bc, self.algo, self.start,
-                      self.seed, self.n_therm, self.n_meas, self.n_cor, self.or_sweeps, self.normalize_every)
+                      self.seed, self.n_therm, self.n_meas, self.n_cor, self.or_sweeps, self.normalize_every,
+                      self.fp64_twin)
 
     @property
     def V(self) -> int:
@@ -192,6 +196,11 @@ class Result:
     P_err: float
     accept: float
     link_updates: int
+    P_fp64_twin: float = 0.0  # jobs with fp64_twin: the FP64 twin's plaquette and binned errors
+    P_fp64_twin_err: float = 0.0
+    P_err_binned: float = 0.0
+    P_diff: float = 0.0  # P_mean - P_fp64_twin
+    P_diff_sigma: float = 0.0  # combined binned error of P_diff
     E: np.ndarray | None = field(default=None, repr=False)
     P: np.ndarray | None = field(default=None, repr=False)
 
@@ -231,7 +240,8 @@ class Server:
         n = C.c_int()
         rc = self.lib.su2_server_poll(self.h, out, max_results, C.byref(n))
         res = [Result(r.id, r.status, r.t_submit, r.t_admit, r.t_done, r.E_mean, r.E_err, r.P_mean, r.P_err,
-                      r.accept, r.link_updates) for r in out[: n.value]]
+                      r.accept, r.link_updates, r.P_fp64_twin, r.P_fp64_twin_err, r.P_err_binned, r.P_diff,
+                      r.P_diff_sigma) for r in out[: n.value]]
         if rc != L.SU2_OK:
             err = L.Su2Error(rc, "server_poll", self.lib.su2_last_error().decode(errors="replace"))
             err.results = res

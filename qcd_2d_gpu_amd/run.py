@@ -23,6 +23,13 @@ class RunResult:
     seconds_meas: float
     link_updates: int
     Nterms: int
+    # FP32-link runs with fp64_twin=True: the FP64 twin's plaquette, the
+    # 20-bin errors, the difference and its combined error (else 0)
+    P_fp64_twin: float = 0.0
+    P_fp64_twin_err: float = 0.0
+    P_err_binned: float = 0.0
+    P_diff_fp32_fp64: float = 0.0
+    P_diff_sigma: float = 0.0
 
 
 def run(Lt: int = 16, Lx: int = 16, Ly: int = 16, beta: float = 3.0, boundary: int = L.BC_PERIODIC,
@@ -31,7 +38,7 @@ def run(Lt: int = 16, Lx: int = 16, Ly: int = 16, beta: float = 3.0, boundary: i
         precision: int = L.PREC_F64, normalize_every: int = 100, seed: int = 1234, readconf: int = 0,
         sconf: int = 1, save: int = 1, outdir: str = ".", nsave: str = "1", nconfr: str = "1",
         nconfs: str = "1", verbose: bool = False, device: int = 0, N_block: int = 20,
-        Nblock_term: int = 10) -> RunResult:
+        Nblock_term: int = 10, fp64_twin: bool = False) -> RunResult:
     """Run one simulation with the reference's parameters (kernel.cu:10-46
     as keyword arguments).  sconf/save follow the reference: 0 = write."""
     lib = L.load()
@@ -41,7 +48,7 @@ def run(Lt: int = 16, Lx: int = 16, Ly: int = 16, beta: float = 3.0, boundary: i
                      N_cor=N_cor, Nsave=Nsave, epsilon=epsilon, upd=upd, W=W, or_sweeps=or_sweeps,
                      precision=precision, normalize_every=normalize_every, seed=seed, readconf=readconf,
                      sconf=sconf, save=save, verbose=int(verbose), device=device, N_block=N_block,
-                     Nblock_term=Nblock_term).items():
+                     Nblock_term=Nblock_term, fp64_twin=int(fp64_twin)).items():
         setattr(cfg, k, v)
     keep = [s.encode() for s in (outdir, nsave, nconfr, nconfs)]
     cfg.outdir, cfg.nsave, cfg.nconfr, cfg.nconfs = keep
@@ -52,4 +59,5 @@ def run(Lt: int = 16, Lx: int = 16, Ly: int = 16, beta: float = 3.0, boundary: i
     res.plaquette = pl.ctypes.data_as(C.POINTER(C.c_double))
     L.check(lib.su2_run(C.byref(cfg), C.byref(res)), "su2_run")
     return RunResult(en, pl, res.E_mean, res.E_err, res.P_mean, res.P_err, res.accept, res.det_mean,
-                     res.seconds_therm, res.seconds_meas, int(res.link_updates), res.Nterms)
+                     res.seconds_therm, res.seconds_meas, int(res.link_updates), res.Nterms, res.P_fp64_twin,
+                     res.P_fp64_twin_err, res.P_err_binned, res.P_diff_fp32_fp64, res.P_diff_sigma)

@@ -87,11 +87,13 @@ def test_invalid_arguments_rejected_before_launch():
 
 
 def test_couplings_that_would_never_finish_or_give_nan_are_rejected():
-    """beta <= 0 would keep the Kennedy-Pendleton loop rejecting for ever,
+    """beta <= 0 would keep the Kennedy-Pendleton loop rejecting for ever and
+    beta below 0.01 nearly so (about 1.26 (beta |A|)^1.5 accepted per trial),
     |eps| > 1 makes the Metropolis proposal NaN: refused on the host."""
     lib = q.load_library()
     fake = C.c_void_p(0x1000)
     for beta, eps, algo, ok in [(0.0, 0.1, 1, False), (-1.0, 0.1, 1, False), (float("nan"), 0.1, 1, False),
+                                (0.0099, 0.1, 1, False), (1e-6, 0.1, 1, False), (0.01, 0.1, 1, True),
                                 (float("inf"), 0.1, 0, False), (2.0, 1.5, 0, False), (2.0, -0.1, 0, False),
                                 (0.0, 0.5, 0, True), (-1.0, 0.5, 0, True), (2.0, 1.0, 0, True)]:
         l = q.make_lattice(8, 8, 8, 1.0)
@@ -108,6 +110,8 @@ def test_couplings_that_would_never_finish_or_give_nan_are_rejected():
             assert b"beta" in msg or b"epsilon" in msg, (beta, eps, algo, msg)
     j = q.Job(8, 8, 8, 0.0)
     assert lib.su2_job_bytes(C.byref(j.c()), 0) == 0
+    assert lib.su2_job_bytes(C.byref(q.Job(8, 8, 8, 0.0099).c()), 0) == 0
+    assert lib.su2_job_bytes(C.byref(q.Job(8, 8, 8, 0.01).c()), 0) > 0
     j = q.Job(8, 8, 8, 2.0, algo=q.ALGO_METROPOLIS, epsilon=1.5)
     assert lib.su2_job_bytes(C.byref(j.c()), 0) == 0
 
@@ -198,3 +202,19 @@ def test_job_bytes_rejects_invalid_jobs(Lt, Lx, Ly, bc, algo, start, n_therm, n_
         # field + RNG + acceptance + partials + two series, each 256-byte aligned
         assert b64 >= 96 * V + 12 * V + 4 * V + 16 * n_meas and b64 % 256 == 0
         assert b32 >= 48 * V + 12 * V + 4 * V and b32 < b64
+
+
+def test_server_creation_error_comes_back_from_its_thread():
+    """The server's thread creates its engine; a configuration the engine
+    refuses (before any device call) fails su2_server_create with the
+    engine's status and message on the caller's thread, and leaves no
+    server behind."""
+    from qcd_2d_gpu_amd import engine as E
+
+    lib = q.load_library()
+    for cfg, word in ((E.Su2EngineConfig(5, 0, 4, 1, 1 << 20, 0, 0, 0), b"precision"),
+                      (E.Su2EngineConfig(0, 0, 4, 1, 1 << 20, 0, 100, 0), b"block")):
+        h = C.c_void_p(0x1234)
+        assert lib.su2_server_create(C.byref(cfg), C.byref(h)) == L.SU2_EINVAL
+        assert h.value is None
+        assert word in lib.su2_last_error(), lib.su2_last_error()

@@ -276,3 +276,38 @@ def test_c2_long_run_matches_checker(gpu, oracle_mod, tmp_path):
         assert eg > 0 and abs(mg - mc) <= 1e-12 * abs(mc) and abs(mg - mc) < 0.01 * eg
     # the thermalised plaquette of the weak-coupling side: 1 - <P> close to 1/beta (3-D SU(2))
     assert 0.75 < r.P_mean < 0.85
+
+
+def test_fp32_run_states_its_error_against_an_fp64_twin(gpu, tmp_path):
+    """su2_run with FP32 links and fp64_twin (the boundary + Metropolis +
+    precision configuration, PMC, 200 + 600 sweeps): the FP32 series is the
+    one the run gives without the twin; the twin's plaquette and 20-bin
+    error are those of the same run with FP64 links (errors to 1e-9: the
+    bin means are summed in another order than numpy's); the stated
+    difference and combined error follow, are written into Energy_av_ and
+    the FP32 result is within 3 combined errors of FP64."""
+    q = gpu
+    print("[driver] FP32 run with FP64 twin, 32x32x48 PMC Metropolis", flush=True)
+    kw = dict(Lt=32, Lx=32, Ly=48, beta=4.0, boundary=q.BC_PMC, algo=q.ALGO_METROPOLIS, init=q.START_HOT,
+              epsilon=0.5, upd=4, N_term=200, N=600, seed=4321)
+    r = q.run(precision=q.PREC_F32, fp64_twin=True, save=0, outdir=str(tmp_path), **kw)
+    alone = q.run(precision=q.PREC_F32, **kw)
+    r64 = q.run(precision=q.PREC_F64, **kw)
+    assert np.array_equal(r.plaquette.view(np.uint64), alone.plaquette.view(np.uint64))
+    m64, e64 = q.stats.binned_error(r64.plaquette, 20)
+    m32, e32 = q.stats.binned_error(r.plaquette, 20)
+    assert abs(r.P_fp64_twin - m64) <= 1e-12 and abs(r.P_fp64_twin_err - e64) <= 1e-9 * e64
+    assert abs(r.P_err_binned - e32) <= 1e-9 * e32
+    assert abs(r.P_diff_fp32_fp64 - (m32 - m64)) <= 1e-12
+    assert abs(r.P_diff_sigma - np.hypot(e32, e64)) <= 1e-9 * r.P_diff_sigma
+    z = r.P_diff_fp32_fp64 / r.P_diff_sigma
+    assert abs(z) < 3.0, z
+    lines = (tmp_path / "Energy_av_32_32_48_B_4.00_pmc_1.txt").read_text().splitlines()
+    assert len(lines) == 5 and lines[4].startswith("FP64_twin_plaquette ")
+    f = lines[4].split()
+    assert f[3] == "FP32_minus_FP64" and f[5] == "sigma" and f[7] == "diff_over_sigma"
+    assert abs(float(f[1]) - m64) < 1e-9 and abs(float(f[4]) - (m32 - m64)) < 1e-9
+    assert abs(float(f[-1]) - z) < 1e-3
+    # without the twin the file keeps the reference's four lines
+    q.run(precision=q.PREC_F32, save=0, outdir=str(tmp_path), nsave="2", **kw)
+    assert len((tmp_path / "Energy_av_32_32_48_B_4.00_pmc_2.txt").read_text().splitlines()) == 4

@@ -341,3 +341,44 @@ def test_server_c5_poisson_stream(gpu, oracle_mod):
         assert any(res[i].t_admit >= first_done for i in ids[:burst]), "no request waited for memory"
     finally:
         srv.close()
+
+
+def test_server_fp32_jobs_with_fp64_twins(gpu):
+    """An FP32-link server with requests that ask for an FP64 twin and
+    requests that do not: every FP32 series equals the job run alone; each
+    twin's plaquette and binned error equal the same job run with FP64 links
+    through su2_run, and the stated difference and combined error follow
+    from the two series."""
+    q = gpu
+    _say("[server] FP32 jobs with FP64 twins")
+    rng = np.random.default_rng(31)
+    jobs = c5_requests(q, 6, rng)
+    for k, j in enumerate(jobs):
+        j.n_therm, j.n_meas = 20, 60
+        j.fp64_twin = int(k % 3 != 2)
+    srv = q.Server(precision=q.PREC_F32, max_jobs=16, budget_bytes=256 << 20, use_graph=True)
+    try:
+        ids = [srv.submit(j) for j in jobs]
+        srv.drain()
+        series = {i: srv.series(i) for i in ids}
+        res = {r.id: r for r in srv.poll()}
+        for i, j in zip(ids, jobs):
+            r = res[i]
+            assert r.status == 0
+            kw = dict(Lt=j.Lt, Lx=j.Lx, Ly=j.Ly, beta=j.beta, boundary=j.bc, algo=j.algo, init=q.START_HOT,
+                      N=j.n_meas, N_term=j.n_therm, seed=j.seed)
+            r32 = q.run(precision=q.PREC_F32, **kw)
+            assert np.array_equal(series[i][1].view(np.uint64), r32.plaquette.view(np.uint64)), f"request {i}"
+            if not j.fp64_twin:
+                assert r.P_fp64_twin == 0.0 and r.P_diff_sigma == 0.0
+                continue
+            r64 = q.run(precision=q.PREC_F64, **kw)
+            m64, e64 = q.stats.binned_error(r64.plaquette, 20)
+            m32, e32 = q.stats.binned_error(r32.plaquette, 20)
+            assert abs(r.P_fp64_twin - m64) <= 1e-12 and abs(r.P_fp64_twin_err - e64) <= 1e-9 * e64
+            assert abs(r.P_err_binned - e32) <= 1e-9 * e32
+            assert abs(r.P_diff - (m32 - m64)) <= 1e-12
+            assert abs(r.P_diff_sigma - np.hypot(e32, e64)) <= 1e-9 * r.P_diff_sigma
+            _say(f"[server]   request {i} {j.Lt}x{j.Lx}x{j.Ly}: FP32 - FP64 = {r.P_diff / r.P_diff_sigma:+.2f} sigma")
+    finally:
+        srv.close()
