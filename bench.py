@@ -630,6 +630,52 @@ def kernel_sources_digest():
     return h.hexdigest()[:16]
 
 
+# The probe shapes that came out best in stream_ceiling's full sweep on
+# MI355X boxes (unroll, nontemporal, workgroups per CU; 0 = one item per
+# thread): the default run measures only these and keeps the best.
+CEILING_SHAPES = {("dram", 1): ((1, 1, 0), (2, 1, 0)), ("dram", 4): ((4, 1, 0), (2, 1, 0)),
+                  ("cache", 1): ((1, 0, 4), (1, 0, 0), (8, 1, 16)), ("cache", 4): ((1, 0, 0), (2, 0, 8), (1, 0, 2))}
+
+
+def stream_ceiling_quick():
+    """The streaming ceilings of stream_ceiling at their best shapes only:
+    {regime_readsN: GB/s}; 'cache' (40 MiB arrays, Infinity-Cache resident)
+    bounds the 128^3 sweep (235 MB working set), 'dram' (512 MiB arrays) the
+    256^3 one."""
+    import qcd_2d_gpu_amd as q
+    from qcd_2d_gpu_amd import _lib as L
+
+    lib = q.load_library()
+    stream = C.c_void_p()
+    L.check(lib.su2_stream_create(C.byref(stream)), "stream_create")
+    a, b = C.c_void_p(), C.c_void_p()
+    L.check(lib.su2_event_create(C.byref(a)), "event")
+    L.check(lib.su2_event_create(C.byref(b)), "event")
+    ms = C.c_float()
+    out = {}
+    for (regime, reads), shapes in CEILING_SHAPES.items():
+        region = (512 << 20) if regime == "dram" else (40 << 20)
+        src, dst = q.DeviceBuffer(reads * region), q.DeviceBuffer(region)
+        src.zero(stream)
+        best = 0.0
+        for unroll, nt, bpc in shapes:
+            for _ in range(2):
+                L.check(lib.su2_event_record(a, stream), "record")
+                for _ in range(10):
+                    L.check(lib.su2_stream_probe(dst.ptr, src.ptr, region, reads, unroll, nt, bpc * NCU, stream),
+                            "stream_probe")
+                L.check(lib.su2_event_record(b, stream), "record")
+                L.check(lib.su2_event_elapsed_ms(a, b, C.byref(ms)), "elapsed")
+                best = max(best, (reads + 1) * region / (ms.value * 1e-3 / 10) / 1e9)
+        src.free()
+        dst.free()
+        out[f"{regime}_reads{reads}"] = best
+    lib.su2_event_destroy(a)
+    lib.su2_event_destroy(b)
+    lib.su2_stream_destroy(stream)
+    return out
+
+
 def load_profile_value(fname, key, profiles_dir=None):
     """The number on the line "key: N" of a committed rocprofv3 summary
     under profiles/, or None when the summary is missing, lacks the key, or
@@ -711,8 +757,14 @@ def main():
             extra[name] = {"error": f"{type(e).__name__}: {e}"}
             print(f"bench.py: {name} failed: {e}", file=sys.stderr)
 
+    section("stream_ceiling", stream_ceiling_quick)
+    ceil = extra.get("stream_ceiling") if "error" not in extra.get("stream_ceiling", {"error": 1}) else None
     if args.large_L:
         section("large_lattice_fp64", lambda: large_lattice_bench(args))
+        if ceil and "heatbath" in extra["large_lattice_fp64"]:
+            for k in ("heatbath", "overrelax"):
+                d = extra["large_lattice_fp64"][k]
+                d["frac_of_stream_ceiling"] = d["algorithmic_gbs"] / ceil["dram_reads4"]
     if args.large_L and args.shard_shares:
         section("shard_share_fp64", shard_share_bench)
     if args.small_configs and args.cpu_steps != 0:
@@ -762,7 +814,12 @@ def main():
                      "traffic_gbs": (load_traffic() / (r["hb_ms"] * 1e-3) / 1e9) if load_traffic() else None,
                      "traffic_source": "profiles/hb_sweep_128cube_traffic.txt" if load_traffic() else
                      "none: no rocprofv3 summary made from the current kernel sources "
-                     f"({kernel_sources_digest()})"},
+                     f"({kernel_sources_digest()})",
+                     "stream_ceiling_gbs": ceil["cache_reads4"] if ceil else None,
+                     "frac_of_stream_ceiling": hb_gbs / ceil["cache_reads4"] if ceil else None,
+                     "stream_ceiling_is": "su2_stream_probe, 4 arrays in and 1 out of 40 MiB each (resident in "
+                                          "the Infinity Cache like the 235 MB of the 128^3 lattice), measured in "
+                                          "this run"},
         "overrelax_launch": {"avg_launch_us": r["or_ms"] * 1e3, "achieved_gbs": or_gbs,
                              "updates_per_s_kernel": per_launch_updates / (r["or_ms"] * 1e-3)},
         "fp64_rate": fp64_rate(r, per_launch_updates),
