@@ -39,6 +39,17 @@ BYTES_HB = 240.0  # per heat-bath link update, FP64 links
 BYTES_OR = 224.0  # per over-relaxation link update, FP64 links
 BYTES_METRO = 272.0  # per Metropolis link update (any number of hits), FP64 links
 BYTES_EN_AV = 96.0  # per site and measurement, FP64 links
+# the same with FP32 link storage (16 B links; the 48 B of RNG state per
+# heat-bath and Metropolis update are unchanged)
+BYTES_F32 = {"hb": 144.0, "or": 112.0, "metro": 160.0, "en_av": 48.0}
+
+
+def update_bytes(precision):
+    """Algorithmic bytes per link update (per site for en_av) of the launches
+    with the given link storage."""
+    if precision == 1:
+        return dict(BYTES_F32)
+    return {"hb": BYTES_HB, "or": BYTES_OR, "metro": BYTES_METRO, "en_av": BYTES_EN_AV}
 
 
 def parse():
@@ -743,9 +754,13 @@ def main():
     ms_step = r["total_ms"] / args.steps
     ups = 2 * 3 * V / (ms_step * 1e-3)
     per_launch_updates = V / 2
-    hb_gbs = BYTES_HB * per_launch_updates / (r["hb_ms"] * 1e-3) / 1e9
-    or_gbs = BYTES_OR * per_launch_updates / (r["or_ms"] * 1e-3) / 1e9
-    step_bytes = 3 * V * (BYTES_HB + BYTES_OR) + 96.0 * V
+    ub = update_bytes(args.precision)
+    # the committed traffic and flop summaries are of the 128^3 FP64 workload
+    headline = args.precision == 0 and args.L == 128
+    traffic = load_traffic() if headline else None
+    hb_gbs = ub["hb"] * per_launch_updates / (r["hb_ms"] * 1e-3) / 1e9
+    or_gbs = ub["or"] * per_launch_updates / (r["or_ms"] * 1e-3) / 1e9
+    step_bytes = 3 * V * (ub["hb"] + ub["or"]) + ub["en_av"] * V
     extra = {}
 
     def section(name, fn):
@@ -807,14 +822,14 @@ def main():
                    "start": "Haar-random SU(2) links, seed 7",
                    "links": "FP64" if args.precision == 0 else "FP32", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": hb_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hb_gbs / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "frac": hb_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "heat-bath sweep launch (one direction x parity)",
-                     "bytes_per_update": BYTES_HB, "updates_per_launch": per_launch_updates,
+                     "bytes_per_update": ub["hb"], "updates_per_launch": per_launch_updates,
                      "avg_launch_us": r["hb_ms"] * 1e3,
-                     "traffic_gbs": (load_traffic() / (r["hb_ms"] * 1e-3) / 1e9) if load_traffic() else None,
-                     "traffic_source": "profiles/hb_sweep_128cube_traffic.txt" if load_traffic() else
-                     "none: no rocprofv3 summary made from the current kernel sources "
-                     f"({kernel_sources_digest()})",
+                     "traffic_gbs": (traffic / (r["hb_ms"] * 1e-3) / 1e9) if traffic else None,
+                     "traffic_source": "profiles/hb_sweep_128cube_traffic.txt" if traffic else
+                     ("none: no rocprofv3 summary made from the current kernel sources "
+                      f"({kernel_sources_digest()})" if headline else "none: the summary is of the 128^3 FP64 run"),
                      "stream_ceiling_gbs": ceil["cache_reads4"] if ceil else None,
                      "frac_of_stream_ceiling": hb_gbs / ceil["cache_reads4"] if ceil else None,
                      "stream_ceiling_is": "su2_stream_probe, 4 arrays in and 1 out of 40 MiB each (resident in "
@@ -822,14 +837,14 @@ def main():
                                           "this run"},
         "overrelax_launch": {"avg_launch_us": r["or_ms"] * 1e3, "achieved_gbs": or_gbs,
                              "updates_per_s_kernel": per_launch_updates / (r["or_ms"] * 1e-3)},
-        "fp64_rate": fp64_rate(r, per_launch_updates),
+        "fp64_rate": fp64_rate(r, per_launch_updates) if headline else None,
         "metropolis_launch": {"hits": 4, "epsilon": 0.5, "avg_launch_us": r["me_ms"] * 1e3,
-                              "achieved_gbs": BYTES_METRO * per_launch_updates / (r["me_ms"] * 1e-3) / 1e9,
-                              "frac_of_hbm_peak": BYTES_METRO * per_launch_updates / (r["me_ms"] * 1e-3) / 1e9
+                              "achieved_gbs": ub["metro"] * per_launch_updates / (r["me_ms"] * 1e-3) / 1e9,
+                              "frac_of_hbm_peak": ub["metro"] * per_launch_updates / (r["me_ms"] * 1e-3) / 1e9
                               / HBM_PEAK_GBS},
         "en_av_launch": {"avg_us": r["en_ms"] * 1e3,
-                         "achieved_gbs": BYTES_EN_AV * V / (r["en_ms"] * 1e-3) / 1e9,
-                         "frac_of_hbm_peak": BYTES_EN_AV * V / (r["en_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                         "achieved_gbs": ub["en_av"] * V / (r["en_ms"] * 1e-3) / 1e9,
+                         "frac_of_hbm_peak": ub["en_av"] * V / (r["en_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "step_algorithmic_gbs": step_bytes / (ms_step * 1e-3) / 1e9,
         "heatbath_updates_per_s_kernel": per_launch_updates / (r["hb_ms"] * 1e-3),
         "observables": {"energy": r["E"], "plaquette": r["P"]},
