@@ -71,6 +71,9 @@ def load() -> C.CDLL:
         "su2_product_tc": (None, [P, P, P]),
         "su2_product_tc1": (None, [P, P, P]),
         "su2_product_tc2": (None, [P, P, P]),
+        "su2_productf": (None, [P, P, P]),
+        "su2_product_tcf": (None, [P, P, P]),
+        "su2_product_tc2f": (None, [P, P, P]),
         "su2_prod_trace": (D, [P, P]),
         "su2_det": (D, [P]),
     }

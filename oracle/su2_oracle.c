@@ -37,7 +37,9 @@
  * gives the same values.
  *
  * FP32-link mode (an extension of the build): every store into the field is
- * rounded to float; all arithmetic stays in double.
+ * rounded to float, and the three-link paths of each staple are multiplied
+ * in float (path_up/path_dn); the staple sum, the updates, reunitarisation
+ * and the observables stay in double.
  */
 #include <float.h>
 #include <math.h>
@@ -472,73 +474,121 @@ static offs make_offs(const orc_lattice *l, int parity, int pos_y, int pos_x, in
     return o;
 }
 
-/* y links: x and t plaquettes (mc_updates.h:155-176, 212-232) */
-static void staple_y(const double *phi, int self, int other, const offs *o, int dg, double *A)
+/* FP32-link extension: the three-link paths of a staple are multiplied in
+ * float (fmaf in the fused pattern of su2_product and its variants) and each
+ * path is added to the double staple sum.  The field holds floats in
+ * doubles, so the narrowing below is exact. */
+static void qf_load(const double *a, float *f)
 {
-    double aux[4], aux2[4];
+    for (int i = 0; i < 4; i++) f[i] = (float)a[i];
+}
+
+void su2_productf(const float *A, const float *B, float *C) /* C = A B */
+{
+    float c0 = fmaf(-A[3], B[3], fmaf(-A[2], B[2], fmaf(-A[1], B[1], A[0] * B[0])));
+    float c1 = fmaf(-A[3], B[2], fmaf(A[2], B[3], fmaf(A[0], B[1], A[1] * B[0])));
+    float c2 = fmaf(-A[1], B[3], fmaf(A[0], B[2], fmaf(A[3], B[1], A[2] * B[0])));
+    float c3 = fmaf(A[1], B[2], fmaf(A[0], B[3], fmaf(-A[2], B[1], A[3] * B[0])));
+    C[0] = c0;
+    C[1] = c1;
+    C[2] = c2;
+    C[3] = c3;
+}
+
+void su2_product_tcf(const float *A, const float *B, float *C) /* C = A B^dag */
+{
+    float c0 = fmaf(A[3], B[3], fmaf(A[2], B[2], fmaf(A[1], B[1], A[0] * B[0])));
+    float c1 = fmaf(A[3], B[2], fmaf(-A[2], B[3], fmaf(-A[0], B[1], A[1] * B[0])));
+    float c2 = fmaf(A[1], B[3], fmaf(-A[0], B[2], fmaf(-A[3], B[1], A[2] * B[0])));
+    float c3 = fmaf(-A[1], B[2], fmaf(-A[0], B[3], fmaf(A[2], B[1], A[3] * B[0])));
+    C[0] = c0;
+    C[1] = c1;
+    C[2] = c2;
+    C[3] = c3;
+}
+
+void su2_product_tc2f(const float *A, const float *B, float *C) /* C = A^dag B^dag */
+{
+    float c0 = fmaf(-A[3], B[3], fmaf(-A[2], B[2], fmaf(-A[1], B[1], A[0] * B[0])));
+    float c1 = fmaf(-A[3], B[2], fmaf(A[2], B[3], fmaf(-A[0], B[1], -A[1] * B[0])));
+    float c2 = fmaf(-A[1], B[3], fmaf(-A[0], B[2], fmaf(A[3], B[1], -A[2] * B[0])));
+    float c3 = fmaf(A[1], B[2], fmaf(-A[0], B[3], fmaf(-A[2], B[1], -A[3] * B[0])));
+    C[0] = c0;
+    C[1] = c1;
+    C[2] = c2;
+    C[3] = c3;
+}
+
+/* A += [B C^dag] D^dag, the "positive" path of a staple */
+static void path_up(const double *B, const double *C, const double *D, int f32, double *A)
+{
+    if (f32) {
+        float b[4], c[4], d[4], aux[4], aux2[4];
+        qf_load(B, b);
+        qf_load(C, c);
+        qf_load(D, d);
+        su2_product_tcf(b, c, aux);
+        su2_product_tcf(aux, d, aux2);
+        for (int i = 0; i < 4; i++) A[i] = A[i] + (double)aux2[i];
+    } else {
+        double aux[4], aux2[4];
+        su2_product_tc(B, C, aux);
+        su2_product_tc(aux, D, aux2);
+        su2_sum(A, aux2);
+    }
+}
+
+/* A += [B^dag C^dag] D, the "negative" path of a staple */
+static void path_dn(const double *B, const double *C, const double *D, int f32, double *A)
+{
+    if (f32) {
+        float b[4], c[4], d[4], aux[4], aux2[4];
+        qf_load(B, b);
+        qf_load(C, c);
+        qf_load(D, d);
+        su2_product_tc2f(b, c, aux);
+        su2_productf(aux, d, aux2);
+        for (int i = 0; i < 4; i++) A[i] = A[i] + (double)aux2[i];
+    } else {
+        double aux[4], aux2[4];
+        su2_product_tc2(B, C, aux);
+        su2_product(aux, D, aux2);
+        su2_sum(A, aux2);
+    }
+}
+
+/* y links: x and t plaquettes (mc_updates.h:155-176, 212-232) */
+static void staple_y(const double *phi, int self, int other, const offs *o, int dg, int f32, double *A)
+{
     su2_zero(A);
-    su2_product_tc(phi + other + o->yp + dg, phi + other + o->xp, aux);
-    su2_product_tc(aux, phi + self + dg, aux2);
-    su2_sum(A, aux2);
-    su2_product_tc2(phi + self + o->yp + dg + o->xm, phi + other + o->xm, aux);
-    su2_product(aux, phi + other + o->xm + dg, aux2);
-    su2_sum(A, aux2);
-    su2_product_tc(phi + other + o->yp + 2 * dg, phi + other + o->tp, aux);
-    su2_product_tc(aux, phi + self + 2 * dg, aux2);
-    su2_sum(A, aux2);
-    su2_product_tc2(phi + self + o->yp + 2 * dg + o->tm, phi + other + o->tm, aux);
-    su2_product(aux, phi + other + o->tm + 2 * dg, aux2);
-    su2_sum(A, aux2);
+    path_up(phi + other + o->yp + dg, phi + other + o->xp, phi + self + dg, f32, A);
+    path_dn(phi + self + o->yp + dg + o->xm, phi + other + o->xm, phi + other + o->xm + dg, f32, A);
+    path_up(phi + other + o->yp + 2 * dg, phi + other + o->tp, phi + self + 2 * dg, f32, A);
+    path_dn(phi + self + o->yp + 2 * dg + o->tm, phi + other + o->tm, phi + other + o->tm + 2 * dg, f32, A);
 }
 
 /* x links: y and t plaquettes (mc_updates.h:265-285, 323-343; PMC skips as
  * mc_updates_pmc.h:33-45, 96-108) */
 static void staple_x(const double *phi, int self, int other, const offs *o, int dg, int skip_yup, int skip_ydn,
-                     double *A)
+                     int f32, double *A)
 {
-    double aux[4], aux2[4];
     su2_zero(A);
-    if (!skip_yup) {
-        su2_product_tc(phi + other + o->xp, phi + other + o->yp + dg, aux);
-        su2_product_tc(aux, phi + self, aux2);
-        su2_sum(A, aux2);
-    }
-    if (!skip_ydn) {
-        su2_product_tc2(phi + self + o->xp + o->ym, phi + other + o->ym + dg, aux);
-        su2_product(aux, phi + other + o->ym, aux2);
-        su2_sum(A, aux2);
-    }
-    su2_product_tc(phi + other + o->xp + 2 * dg, phi + other + o->tp + dg, aux);
-    su2_product_tc(aux, phi + self + 2 * dg, aux2);
-    su2_sum(A, aux2);
-    su2_product_tc2(phi + self + o->xp + 2 * dg + o->tm, phi + other + o->tm + dg, aux);
-    su2_product(aux, phi + other + o->tm + 2 * dg, aux2);
-    su2_sum(A, aux2);
+    if (!skip_yup) path_up(phi + other + o->xp, phi + other + o->yp + dg, phi + self, f32, A);
+    if (!skip_ydn) path_dn(phi + self + o->xp + o->ym, phi + other + o->ym + dg, phi + other + o->ym, f32, A);
+    path_up(phi + other + o->xp + 2 * dg, phi + other + o->tp + dg, phi + self + 2 * dg, f32, A);
+    path_dn(phi + self + o->xp + 2 * dg + o->tm, phi + other + o->tm + dg, phi + other + o->tm + 2 * dg, f32, A);
 }
 
 /* t links: y and x plaquettes (mc_updates.h:377-397, 437-457; PMC skips as
  * mc_updates_pmc.h:156-167, 219-230) */
 static void staple_t(const double *phi, int self, int other, const offs *o, int dg, int skip_yup, int skip_ydn,
-                     double *A)
+                     int f32, double *A)
 {
-    double aux[4], aux2[4];
     su2_zero(A);
-    if (!skip_yup) {
-        su2_product_tc(phi + other + o->tp, phi + other + o->yp + 2 * dg, aux);
-        su2_product_tc(aux, phi + self, aux2);
-        su2_sum(A, aux2);
-    }
-    if (!skip_ydn) {
-        su2_product_tc2(phi + self + o->tp + o->ym, phi + other + o->ym + 2 * dg, aux);
-        su2_product(aux, phi + other + o->ym, aux2);
-        su2_sum(A, aux2);
-    }
-    su2_product_tc(phi + other + o->tp + dg, phi + other + o->xp + 2 * dg, aux);
-    su2_product_tc(aux, phi + self + dg, aux2);
-    su2_sum(A, aux2);
-    su2_product_tc2(phi + self + o->tp + dg + o->xm, phi + other + o->xm + 2 * dg, aux);
-    su2_product(aux, phi + other + o->xm + dg, aux2);
-    su2_sum(A, aux2);
+    if (!skip_yup) path_up(phi + other + o->tp, phi + other + o->yp + 2 * dg, phi + self, f32, A);
+    if (!skip_ydn) path_dn(phi + self + o->tp + o->ym, phi + other + o->ym + 2 * dg, phi + other + o->ym, f32, A);
+    path_up(phi + other + o->tp + dg, phi + other + o->xp + 2 * dg, phi + self + dg, f32, A);
+    path_dn(phi + self + o->tp + dg + o->xm, phi + other + o->xm + 2 * dg, phi + other + o->xm + dg, f32, A);
 }
 
 /* One (direction mu, parity) launch: MC_update_{even,odd}_{y,x,t} with the
@@ -563,11 +613,11 @@ void orc_mc_update(double *phi, const orc_lattice *l, int *accept, orc_rng *stat
         if (mu != 0 && bc == 1 && at_y0) continue; /* mc_updates_pcc.h:29,83,134,189 */
         int skip_up = (bc == 2) && at_ylast, skip_dn = (bc == 2) && at_y0;
         if (mu == 0)
-            staple_y(phi, self, other, &o, l->dg, A);
+            staple_y(phi, self, other, &o, l->dg, f32, A);
         else if (mu == 1)
-            staple_x(phi, self, other, &o, l->dg, skip_up, skip_dn, A);
+            staple_x(phi, self, other, &o, l->dg, skip_up, skip_dn, f32, A);
         else
-            staple_t(phi, self, other, &o, l->dg, skip_up, skip_dn, A);
+            staple_t(phi, self, other, &o, l->dg, skip_up, skip_dn, f32, A);
         accept[self / l->dt] += link_update(A, phi + self + mu * l->dg, l, state + k, algo, update, f32);
     }
 }

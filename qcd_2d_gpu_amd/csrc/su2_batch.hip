@@ -258,26 +258,8 @@ __global__ __launch_bounds__(1024) void k_ben_av_final(const su2_bhead *__restri
             continue;
         }
         const uint32_t m = (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor;
-        double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
-        for (uint32_t i = threadIdx.x; i < d.en_nb; i += 4 * SU2_EN_FINAL_BLOCK) {
-            /* loads first, as in the single-lattice pass 2 */
-            double ve[4], vp[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t q = i + k * SU2_EN_FINAL_BLOCK, qq = q < d.en_nb ? q : 0u;
-                ve[k] = d.part[2 * qq];
-                vp[k] = d.part[2 * qq + 1];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (i + k * SU2_EN_FINAL_BLOCK < d.en_nb) {
-                    se[k] += ve[k];
-                    sp[k] += vp[k];
-                }
-            }
-        }
-        double te = (se[0] + se[1]) + (se[2] + se[3]);
-        double tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+        double te, tp;
+        su2_en_final_partials(d.part, d.en_nb, &te, &tp); /* the single-lattice order */
         su2_block_sum2(te, tp, SU2_EN_FINAL_BLOCK);
         if (threadIdx.x == 0) {
             const double invV = 1.0 / (2.0 * (double)d.g.Vh);

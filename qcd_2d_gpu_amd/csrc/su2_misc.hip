@@ -102,34 +102,15 @@ __global__ __launch_bounds__(256) void k_en_av_partial(const LinkT *__restrict__
     }
 }
 
-/* Pass 2: one block of 1024 threads adds the partials in a fixed order (four
- * accumulators per thread over a stride of 4 * 1024, then the block
- * sum) and accumulates sum / V into res, res2 (the reference adds into
- * zeroed slots with atomicAdd, functions/observables.h:137-140). */
+/* Pass 2: one block of 1024 threads adds the partials in a fixed order
+ * (su2_en_final_partials, then the block sum) and accumulates sum / V into
+ * res, res2 (the reference adds into zeroed slots with atomicAdd,
+ * functions/observables.h:137-140). */
 __global__ __launch_bounds__(1024) void k_en_av_final(const double *__restrict__ work, uint32_t n, double invV,
                                                       double *__restrict__ res, double *__restrict__ res2)
 {
-    double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t i = threadIdx.x; i < n; i += 4 * SU2_EN_FINAL_BLOCK) {
-        /* the four loads first (an index past n reads partial 0 and is not
-         * added), so their latencies overlap */
-        double ve[4], vp[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t j = i + k * SU2_EN_FINAL_BLOCK, jj = j < n ? j : 0u;
-            ve[k] = work[2 * jj];
-            vp[k] = work[2 * jj + 1];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (i + k * SU2_EN_FINAL_BLOCK < n) {
-                se[k] += ve[k];
-                sp[k] += vp[k];
-            }
-        }
-    }
-    double te = (se[0] + se[1]) + (se[2] + se[3]);
-    double tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+    double te, tp;
+    su2_en_final_partials(work, n, &te, &tp);
     su2_block_sum2(te, tp, SU2_EN_FINAL_BLOCK);
     if (threadIdx.x == 0) {
         *res += te * invV;

@@ -90,4 +90,35 @@ __device__ static inline void su2_block_sum2(double &u, double &v, uint32_t nthr
     }
 }
 
+/* The partial sums of pass 2 for one lattice, per thread of a
+ * SU2_EN_FINAL_BLOCK workgroup: thread t adds partial t + k B + 4 m B (B the
+ * workgroup size) into accumulator k, m ascending, k = 0..3, and returns
+ * (s0 + s1) + (s2 + s3).  Eight partials are loaded per trip, so a lattice
+ * of up to 8 B partials takes one memory round trip; a slot past n reads
+ * partial 0 and is not added. */
+__device__ static inline __attribute__((always_inline)) void su2_en_final_partials(const double *__restrict__ work,
+                                                                                uint32_t n, double *te, double *tp)
+{
+    const uint32_t B = SU2_EN_FINAL_BLOCK;
+    double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = threadIdx.x; i < n; i += 8 * B) {
+        double ve[8], vp[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t j = i + (uint32_t)u * B, jj = j < n ? j : 0u;
+            ve[u] = work[2 * jj];
+            vp[u] = work[2 * jj + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (i + (uint32_t)u * B < n) {
+                se[u & 3] += ve[u];
+                sp[u & 3] += vp[u];
+            }
+        }
+    }
+    *te = (se[0] + se[1]) + (se[2] + se[3]);
+    *tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+}
+
 #endif /* SU2_OBSERVE_H */

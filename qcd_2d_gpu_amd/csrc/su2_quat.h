@@ -131,4 +131,58 @@ SU2_Q_FN su2_q su2_q_normalize_det(su2_q A, double *norm)
     return su2_q_scale(A, inv);
 }
 
+/*
+ * FP32-link staple paths.  With links stored in float, each three-link path
+ * of a staple is multiplied in float, in the fused pattern above with fmaf,
+ * and the path enters the staple sum as a double: the sum of the paths, the
+ * accept tests, reunitarisation and the observables stay double.  The float
+ * operands are the stored links themselves (exact), so the products round
+ * like the FP64 ones with a 24-bit significand.
+ */
+typedef struct su2_qf {
+    float a0, a1, a2, a3;
+} su2_qf;
+
+#define SU2_FMAF(a, b, c) __builtin_fmaf((a), (b), (c))
+
+SU2_Q_FN su2_qf su2_qf_make(float a0, float a1, float a2, float a3)
+{
+    su2_qf r;
+    r.a0 = a0;
+    r.a1 = a1;
+    r.a2 = a2;
+    r.a3 = a3;
+    return r;
+}
+
+/* A B in float (the pattern of su2_q_mul) */
+SU2_Q_FN su2_qf su2_qf_mul(su2_qf A, su2_qf B)
+{
+    return su2_qf_make(SU2_FMAF(-A.a3, B.a3, SU2_FMAF(-A.a2, B.a2, SU2_FMAF(-A.a1, B.a1, A.a0 * B.a0))),
+                       SU2_FMAF(-A.a3, B.a2, SU2_FMAF(A.a2, B.a3, SU2_FMAF(A.a0, B.a1, A.a1 * B.a0))),
+                       SU2_FMAF(-A.a1, B.a3, SU2_FMAF(A.a0, B.a2, SU2_FMAF(A.a3, B.a1, A.a2 * B.a0))),
+                       SU2_FMAF(A.a1, B.a2, SU2_FMAF(A.a0, B.a3, SU2_FMAF(-A.a2, B.a1, A.a3 * B.a0))));
+}
+
+/* A B^dagger in float (the pattern of su2_q_mul_nd) */
+SU2_Q_FN su2_qf su2_qf_mul_nd(su2_qf A, su2_qf B)
+{
+    return su2_qf_make(SU2_FMAF(A.a3, B.a3, SU2_FMAF(A.a2, B.a2, SU2_FMAF(A.a1, B.a1, A.a0 * B.a0))),
+                       SU2_FMAF(A.a3, B.a2, SU2_FMAF(-A.a2, B.a3, SU2_FMAF(-A.a0, B.a1, A.a1 * B.a0))),
+                       SU2_FMAF(A.a1, B.a3, SU2_FMAF(-A.a0, B.a2, SU2_FMAF(-A.a3, B.a1, A.a2 * B.a0))),
+                       SU2_FMAF(-A.a1, B.a2, SU2_FMAF(-A.a0, B.a3, SU2_FMAF(A.a2, B.a1, A.a3 * B.a0))));
+}
+
+/* A^dagger B^dagger in float (the pattern of su2_q_mul_dd) */
+SU2_Q_FN su2_qf su2_qf_mul_dd(su2_qf A, su2_qf B)
+{
+    return su2_qf_make(SU2_FMAF(-A.a3, B.a3, SU2_FMAF(-A.a2, B.a2, SU2_FMAF(-A.a1, B.a1, A.a0 * B.a0))),
+                       SU2_FMAF(-A.a3, B.a2, SU2_FMAF(A.a2, B.a3, SU2_FMAF(-A.a0, B.a1, -A.a1 * B.a0))),
+                       SU2_FMAF(-A.a1, B.a3, SU2_FMAF(-A.a0, B.a2, SU2_FMAF(A.a3, B.a1, -A.a2 * B.a0))),
+                       SU2_FMAF(A.a1, B.a2, SU2_FMAF(-A.a0, B.a3, SU2_FMAF(-A.a2, B.a1, -A.a3 * B.a0))));
+}
+
+/* widening to double, exact */
+SU2_Q_FN su2_q su2_q_of_qf(su2_qf A) { return su2_q_make((double)A.a0, (double)A.a1, (double)A.a2, (double)A.a3); }
+
 #endif /* SU2_QUAT_H */

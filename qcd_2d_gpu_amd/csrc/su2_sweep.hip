@@ -85,11 +85,13 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
      * from the dispatch packet with a vector load whose wait would sit in
      * front of every wavefront's first link and RNG loads */
     const uint32_t h = sweep_block(order, strip, bpp) * nthr + threadIdx.x;
-    if constexpr (UPD == SU2_K_HB) {
+    /* Heat-bath with FP64 links shares the rejection loop across the
+     * workgroup (su2_sweep_site_hb_wg, early RNG load); with FP32 links each
+     * lane runs its own loop (su2_sweep_site), measured 3 % faster per
+     * launch at 128^3 (28.7 against 29.6 us) with FP64 links unchanged. */
+    if constexpr (UPD == SU2_K_HB && sizeof(LinkT) == sizeof(su2_l64)) {
         __shared__ su2_kp_queue q;
-        /* FP64 links: early RNG load (see su2_sweep_site_hb_wg) */
-        su2_sweep_site_hb_wg<MU, BC, LinkT, sizeof(LinkT) == sizeof(su2_l64)>(U, rng, accept, g, ph, parity, h,
-                                                                              h < g.Vh, q, BC, acc_mode);
+        su2_sweep_site_hb_wg<MU, BC, LinkT, true>(U, rng, accept, g, ph, parity, h, h < g.Vh, q, BC, acc_mode);
     } else {
         if (h >= g.Vh) return;
         su2_sweep_site<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, h, BC, acc_mode);

@@ -44,6 +44,11 @@ template <> struct su2_link_io<su2_l64> {
 };
 
 template <> struct su2_link_io<su2_l32> {
+    __device__ static inline __attribute__((always_inline)) su2_qf loadf(const su2_l32 *p)
+    {
+        su2_l32 v = *p;
+        return su2_qf_make(v.a0, v.a1, v.a2, v.a3);
+    }
     __device__ static inline __attribute__((always_inline)) su2_q load(const su2_l32 *p)
     {
         su2_l32 v = *p;
@@ -115,15 +120,27 @@ __device__ static inline __attribute__((always_inline)) int su2_site_staple(cons
         const uint32_t h_pmu_mnu =
             su2_hidx(g, NU == SU2_DIR_T ? tm : tmu, NU == SU2_DIR_X ? xm : xmu, NU == SU2_DIR_Y ? ym : ymu);
 
-        const su2_q u_nu_pmu = io::load(Uo + NU * Vh + h_pmu);
-        const su2_q u_mu_pnu = io::load(Uo + MU * Vh + h_pnu);
-        const su2_q u_nu_n = io::load(Us + NU * Vh + h);
-        const su2_q u_nu_pmu_mnu = io::load(Us + NU * Vh + h_pmu_mnu);
-        const su2_q u_mu_mnu = io::load(Uo + MU * Vh + h_mnu);
-        const su2_q u_nu_mnu = io::load(Uo + NU * Vh + h_mnu);
-
-        su2_q up = su2_q_mul_nd(su2_q_mul_nd(u_nu_pmu, u_mu_pnu), u_nu_n);
-        su2_q dn = su2_q_mul(su2_q_mul_dd(u_nu_pmu_mnu, u_mu_mnu), u_nu_mnu);
+        su2_q up, dn;
+        if constexpr (sizeof(LinkT) == sizeof(su2_l32)) {
+            /* FP32 links: the paths multiplied in float (su2_quat.h) */
+            const su2_qf u_nu_pmu = io::loadf(Uo + NU * Vh + h_pmu);
+            const su2_qf u_mu_pnu = io::loadf(Uo + MU * Vh + h_pnu);
+            const su2_qf u_nu_n = io::loadf(Us + NU * Vh + h);
+            const su2_qf u_nu_pmu_mnu = io::loadf(Us + NU * Vh + h_pmu_mnu);
+            const su2_qf u_mu_mnu = io::loadf(Uo + MU * Vh + h_mnu);
+            const su2_qf u_nu_mnu = io::loadf(Uo + NU * Vh + h_mnu);
+            up = su2_q_of_qf(su2_qf_mul_nd(su2_qf_mul_nd(u_nu_pmu, u_mu_pnu), u_nu_n));
+            dn = su2_q_of_qf(su2_qf_mul(su2_qf_mul_dd(u_nu_pmu_mnu, u_mu_mnu), u_nu_mnu));
+        } else {
+            const su2_q u_nu_pmu = io::load(Uo + NU * Vh + h_pmu);
+            const su2_q u_mu_pnu = io::load(Uo + MU * Vh + h_pnu);
+            const su2_q u_nu_n = io::load(Us + NU * Vh + h);
+            const su2_q u_nu_pmu_mnu = io::load(Us + NU * Vh + h_pmu_mnu);
+            const su2_q u_mu_mnu = io::load(Uo + MU * Vh + h_mnu);
+            const su2_q u_nu_mnu = io::load(Uo + NU * Vh + h_mnu);
+            up = su2_q_mul_nd(su2_q_mul_nd(u_nu_pmu, u_mu_pnu), u_nu_n);
+            dn = su2_q_mul(su2_q_mul_dd(u_nu_pmu_mnu, u_mu_mnu), u_nu_mnu);
+        }
         if (pmc && MU != SU2_DIR_Y && NU == SU2_DIR_Y) {
             /* A is never -0 here, so adding +0 leaves it bit-identical. */
             if (y == g.Ly - 1u) up = su2_q_zero();

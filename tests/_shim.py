@@ -26,6 +26,7 @@ def load():
         "shim_heat_bath": (None, [P, D, P, P]), "shim_metropolis": (I, [P, P, D, D, I, P]),
         "shim_metropolis_accept": (I, [D, D]), "shim_over_relax": (None, [P, P, P]),
         "shim_rand_link": (None, [P, P]), "shim_haar_link": (None, [P, P]), "shim_q_mul": (None, [P, P, P, I]),
+        "shim_qf_mul": (None, [P, P, P, I]),
         "shim_q_scalar": (D, [P, P, I]),
         "shim_fmath_max_ulp": (D, [I, U32, U32, I, C.POINTER(U32), C.POINTER(C.c_longlong)]),
     }

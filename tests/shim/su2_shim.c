@@ -63,6 +63,21 @@ void shim_q_mul(const double *A, const double *B, double *C, int kind)
     st(C, c);
 }
 
+/* float paths of the FP32-link staple; kind 0: A B, 1: A B^dag, 3: A^dag B^dag */
+void shim_qf_mul(const float *A, const float *B, float *C, int kind)
+{
+    su2_qf a = su2_qf_make(A[0], A[1], A[2], A[3]), b = su2_qf_make(B[0], B[1], B[2], B[3]), c;
+    switch (kind) {
+    case 0: c = su2_qf_mul(a, b); break;
+    case 1: c = su2_qf_mul_nd(a, b); break;
+    default: c = su2_qf_mul_dd(a, b); break;
+    }
+    C[0] = c.a0;
+    C[1] = c.a1;
+    C[2] = c.a2;
+    C[3] = c.a3;
+}
+
 /* kind 0: det A, 1: 1/2 Re tr(A B), 2: first component of A B^dag */
 double shim_q_scalar(const double *A, const double *B, int kind)
 {

@@ -120,3 +120,76 @@ def test_products_round_as_the_fused_pattern(oracle_mod):
         assert lib.su2_prod_trace(a.ctypes.data, b.ctypes.data) == retr == shim.shim_q_scalar(
             a.ctypes.data, b.ctypes.data, 1)
         assert nd_re == shim.shim_q_scalar(a.ctypes.data, b.ctypes.data, 2)
+
+
+def _round_f32(x):
+    """A rational rounded to the nearest binary32 value (ties to even),
+    exactly; the values here are normal."""
+    from fractions import Fraction as F
+
+    if x == 0:
+        return np.float32(0.0)
+    a = abs(x)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    if F(2) ** e > a:
+        e -= 1
+    scale = F(2) ** (23 - e)  # 24 significant bits
+    m = a * scale
+    q, r = divmod(m.numerator, m.denominator)
+    if 2 * r > m.denominator or (2 * r == m.denominator and q % 2 == 1):
+        q += 1
+    v = F(q) / scale
+    return np.float32(float(v if x > 0 else -v))
+
+
+def _fused_sum_f32(terms, a, b):
+    """The fused pattern in binary32: first product rounded alone, each later
+    term added with one rounding, modelled with rationals."""
+    from fractions import Fraction as F
+
+    s0, i0, j0 = terms[0]
+    t = _round_f32(F(float(s0 * a[i0])) * F(float(b[j0])))
+    for s, i, j in terms[1:]:
+        t = _round_f32(F(float(s * a[i])) * F(float(b[j])) + F(float(t)))
+    return t
+
+
+def test_float_paths_round_as_the_fused_pattern(oracle_mod):
+    """The FP32-link staple paths: the checker's float products and the
+    product header's (shim) equal, bit for bit, a rational model of the fused
+    pattern rounded to binary32, and the 2x2 matrix products to float
+    accuracy.  A dropped fusion or a term in double is caught: the unfused
+    binary32 evaluation and the double evaluation both differ on these
+    inputs."""
+    lib = oracle_mod.load()
+    shim = _shim.load()
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=(200, 4)).astype(np.float32)
+    B = rng.normal(size=(200, 4)).astype(np.float32)
+    names = {"su2_productf": ("su2_product", 0), "su2_product_tcf": ("su2_product_tc", 1),
+             "su2_product_tc2f": ("su2_product_tc2", 3)}
+    unfused = in_double = 0
+    for name, (dname, kind) in names.items():
+        terms = _TERMS[dname]
+        for i in range(len(A)):
+            a, b = np.ascontiguousarray(A[i]), np.ascontiguousarray(B[i])
+            want = np.array([_fused_sum_f32(t, a, b) for t in terms], dtype=np.float32)
+            c = np.zeros(4, dtype=np.float32)
+            getattr(lib, name)(a.ctypes.data, b.ctypes.data, c.ctypes.data)
+            assert np.array_equal(c.view(np.uint32), want.view(np.uint32)), (name, i)
+            c2 = np.zeros(4, dtype=np.float32)
+            shim.shim_qf_mul(a.ctypes.data, b.ctypes.data, c2.ctypes.data, kind)
+            assert np.array_equal(c2.view(np.uint32), want.view(np.uint32)), (name, i)
+            ad, bd = a.astype(np.float64), b.astype(np.float64)
+            wd = np.array([_fused_sum(t, ad, bd) for t in terms])
+            M = {0: N.quat_to_mat(ad) @ N.quat_to_mat(bd), 1: N.quat_to_mat(ad) @ N.dag(N.quat_to_mat(bd)),
+                 3: N.dag(N.quat_to_mat(ad)) @ N.dag(N.quat_to_mat(bd))}[kind]
+            np.testing.assert_allclose(N.quat_to_mat(c.astype(np.float64)), M, rtol=0, atol=2e-6)
+            in_double += any(np.float64(w) != x for w, x in zip(want, wd))
+            for t, w in zip(terms, want):
+                s0, i0, j0 = t[0]
+                p = np.float32(s0 * a[i0]) * b[j0]
+                for s, i2, j in t[1:]:
+                    p = np.float32(p + np.float32(np.float32(s * a[i2]) * b[j]))
+                unfused += p != w
+    assert unfused > 50 and in_double > 300

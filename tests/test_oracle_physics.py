@@ -11,6 +11,7 @@ full-matrix implementation (tests/_naive.py).
 - equilibrium: strong coupling <P> = u + 4u^5 (u = I2/I1, d = 3), weak
   coupling <P> = 1 - 1/beta + O(1/beta^2), and Metropolis agrees with
   heat-bath within errors;
+- FP32-link mode: the float staple paths keep the action to float accuracy;
 - results do not depend on the number of threads.
 """
 import os
@@ -149,3 +150,34 @@ def test_thread_count_independence(oracle_mod):
     oracle_mod.set_threads(os.cpu_count() or 1)
     assert np.array_equal(res[0][0].view(np.uint64), res[1][0].view(np.uint64))
     assert np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2])
+
+
+@pytest.mark.parametrize("bc", [0, 2])
+def test_fp32_link_staples_multiply_in_float(oracle_mod, bc):
+    """FP32-link mode multiplies the staple paths in float: an over-relaxation
+    launch on a float-valued field keeps the action to float accuracy (a
+    wrong neighbour, sign or orientation in a float path moves it by O(1)),
+    and its links differ from the FP64-mode launch on the same field, rounded
+    to float, by float rounding alone, but do differ (the float paths are
+    taken)."""
+    Lt, Lx, Ly = 4, 6, 8
+    o32 = oracle_mod.Oracle(Lt, Lx, Ly, 2.0, 0.3, 2, f32=True)
+    o32.setup_kernel(3)
+    o32.start(0, bc)
+    o64 = oracle_mod.Oracle(Lt, Lx, Ly, 2.0, 0.3, 2)
+    assert np.array_equal(o32.phi, o32.phi.astype(np.float32).astype(np.float64))
+    differ = 0
+    for mu in range(3):
+        for par in range(2):
+            pmc_act = bc == 2 and mu != 0
+            before = N.action(N.field_to_links(o32.phi, Lt, Lx, Ly), 1.0, pmc_act)
+            o64.phi[:] = o32.phi
+            o32.mc_update(bc, 1, 1, mu, par)
+            o64.mc_update(bc, 1, 1, mu, par)
+            after = N.action(N.field_to_links(o32.phi, Lt, Lx, Ly), 1.0, pmc_act)
+            assert abs(after - before) < 2e-5 * abs(before), (mu, par)
+            # with double paths o32 would equal o64 rounded to float
+            d = np.abs(o32.phi - o64.phi.astype(np.float32).astype(np.float64)).max()
+            assert d < 1e-5, (mu, par)
+            differ += d > 0
+    assert differ == 6
