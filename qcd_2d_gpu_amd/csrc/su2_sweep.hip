@@ -73,7 +73,7 @@ __device__ static inline __attribute__((always_inline)) uint32_t sweep_block(uin
     return (xcd * T + tl) * bpp + s * strip + c;
 }
 
-template <int MU, int BC, int UPD, typename LinkT>
+template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG = false>
 __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *__restrict__ U,
                                                                         uint32_t *__restrict__ rng,
                                                                         int *__restrict__ accept, const su2_kgeom &g,
@@ -91,7 +91,8 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
      * launch at 128^3 (28.7 against 29.6 us) with FP64 links unchanged. */
     if constexpr (UPD == SU2_K_HB && sizeof(LinkT) == sizeof(su2_l64)) {
         __shared__ su2_kp_queue q;
-        su2_sweep_site_hb_wg<MU, BC, LinkT, true>(U, rng, accept, g, ph, parity, h, h < g.Vh, q, BC, acc_mode);
+        su2_sweep_site_hb_wg<MU, BC, LinkT, true, STREAM_RNG>(U, rng, accept, g, ph, parity, h, h < g.Vh, q, BC,
+                                                             acc_mode);
     } else {
         if (h >= g.Vh) return;
         su2_sweep_site<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, h, BC, acc_mode);
@@ -116,13 +117,17 @@ __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *
  * over-relaxation 94) a cap of 5 (100 VGPRs, no spills) timed the same as
  * 4 at 128^3.  FP32 links (half the footprint) are as fast or faster
  * uncapped. */
-template <int MU, int BC, int UPD, typename LinkT>
+template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void
 k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
            int parity, uint32_t order, uint32_t strip, uint32_t bpp, uint32_t nthr, int acc_mode)
 {
-    sweep_body<MU, BC, UPD, LinkT>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr, acc_mode);
+    sweep_body<MU, BC, UPD, LinkT, STREAM_RNG>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr, acc_mode);
 }
+
+/* Bytes of links and RNG state above which a lattice's launches run from
+ * DRAM rather than from the 256 MB Infinity Cache (128^3 FP64: 226 MB). */
+#define SU2_CACHE_RESIDENT_BYTES (256ull << 20)
 
 template <int MU, int BC, int UPD, typename LinkT>
 hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, const su2_kphys &p, int parity,
@@ -141,9 +146,19 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
         order = ORDER_STRIPS;
         strip = w;
     }
-    if constexpr (sizeof(LinkT) == sizeof(su2_l64))
-        hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
-                           p, parity, order, strip, bpp, (uint32_t)block, acc_mode);
+    /* heat-bath of a lattice beyond the Infinity Cache: streaming hints
+     * on the RNG state (su2_sweep_site_hb_wg) */
+    const bool dram = 6ull * g.Vh * sizeof(LinkT) + 24ull * g.Vh > SU2_CACHE_RESIDENT_BYTES;
+    if constexpr (sizeof(LinkT) == sizeof(su2_l64) && UPD == SU2_K_HB) {
+        if (dram)
+            hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, true>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng,
+                               accept, g, p, parity, order, strip, bpp, (uint32_t)block, acc_mode);
+        else
+            hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng,
+                               accept, g, p, parity, order, strip, bpp, (uint32_t)block, acc_mode);
+    } else if constexpr (sizeof(LinkT) == sizeof(su2_l64))
+        hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng,
+                           accept, g, p, parity, order, strip, bpp, (uint32_t)block, acc_mode);
     else
         hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT>), dim3(nb), dim3(block), 0, st, (LinkT *)U, rng, accept, g,
                            p, parity, order, strip, bpp, (uint32_t)block, acc_mode);

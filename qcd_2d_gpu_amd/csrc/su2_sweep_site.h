@@ -224,7 +224,7 @@ struct su2_kp_queue {
     float lf[256];
 };
 
-template <int MU, int BC, typename LinkT, bool EARLY_RNG = true>
+template <int MU, int BC, typename LinkT, bool EARLY_RNG = true, bool STREAM_RNG = false>
 __device__ static inline __attribute__((always_inline)) void
 su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept,
                      const su2_kgeom &g, const su2_kphys &ph, int parity, uint32_t h, int in_range,
@@ -241,12 +241,26 @@ su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__r
      * 1 % either way and keeps the late load; the batched sweep gains 4.5 %
      * with FP32 links and loses 1.6-2 % with FP64 links.  The callers choose
      * accordingly.  (Issuing all twelve neighbour loads before the first
-     * product gained nothing more and cost FP32-link over-relaxation 11 %.) */
+     * product gained nothing more and cost FP32-link over-relaxation 11 %.)
+     *
+     * STREAM_RNG: the stream's state is loaded and stored with nontemporal
+     * hints, for lattices larger than the Infinity Cache, whose launches
+     * run from DRAM: the state is read once and written once per launch,
+     * and the hints keep it from displacing the neighbour links in the
+     * caches.  Measured with FP64 links, alternating on one box: 256^3
+     * heat-bath launch 410 -> 398 us; at 128^3 (resident in the Infinity
+     * Cache, where the next launch re-reads the state from it) the same
+     * hints cost 4.5 %, so they are not used there. */
     uint32_t s[6];
     if constexpr (EARLY_RNG) {
         if (in_range) {
 #pragma unroll
-            for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * g.Vh + h];
+            for (int i = 0; i < 6; ++i) {
+                if constexpr (STREAM_RNG)
+                    s[i] = __builtin_nontemporal_load(&rng[(size_t)i * g.Vh + h]);
+                else
+                    s[i] = rng[(size_t)i * g.Vh + h];
+            }
         }
     }
     su2_q A = su2_q_zero();
@@ -295,7 +309,12 @@ su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__r
         LinkT *dst = U + (size_t)(parity * 3 + MU) * g.Vh + h;
         io::store(dst, su2_kp_finish(A, lf, s));
 #pragma unroll
-        for (int i = 0; i < 6; ++i) rng[(size_t)i * Vh + h] = s[i];
+        for (int i = 0; i < 6; ++i) {
+            if constexpr (STREAM_RNG)
+                __builtin_nontemporal_store(s[i], &rng[(size_t)i * Vh + h]);
+            else
+                rng[(size_t)i * Vh + h] = s[i];
+        }
         /* no-return adds, see su2_sweep_site */
         if (acc_mode == SU2_K_ACC_LAUNCH)
             atomicAdd(&accept[(size_t)parity * Vh + h], 1);
