@@ -86,6 +86,7 @@ def parse():
                     help="requests submitted at once to measure the server's capacity (0, or --serve-requests 0: "
                          "skip)")
     ap.add_argument("--fp32-meas", type=int, default=1000, help="measurements of the FP32-link check (0: skip)")
+    ap.add_argument("--fp32-seeds", type=int, default=16, help="seeds of the FP32-link check per boundary")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the run, launch the plain copy kernel 5 times on buffers of this size "
                          "(reference dispatches for calibrating rocprofv3 byte counters)")
@@ -540,28 +541,57 @@ def fp32_check(args):
     """The boundary + Metropolis + precision configuration: Lt = Lx = 32,
     Ly = 48, PCC and PMC, Metropolis eps 0.5, 4 hits, beta 4.0, hot start, in
     FP64 and with FP32 link storage; plaquette difference in units of the
-    combined binned error."""
+    combined binned error.  Seed 4321 is the configuration's own run; with
+    --fp32-seeds k, seeds 4321 .. 4321 + k - 1 run too (all in one batched
+    engine per precision), and their differences are summarised: how many
+    lie within 1 and 2 combined errors, their chi^2, and the mean difference
+    over the seeds with its error, a test for a bias of FP32 links that is
+    sqrt(k) times more sensitive than one run.  As a control, the same
+    statistic between FP64 runs of two different seeds (pairs of seeds)
+    shows its spread when only the Markov chain differs."""
     import qcd_2d_gpu_amd as q
 
+    seeds = [4321 + k for k in range(max(1, args.fp32_seeds))]
     out = {}
     runs = {}
     for prec in (q.PREC_F64, q.PREC_F32):
-        eng = q.Engine(precision=prec, max_jobs=2, budget_bytes=256 << 20, use_graph=True)
-        slots = {bc: eng.add(q.Job(32, 32, 48, 4.0, bc=bc, algo=q.ALGO_METROPOLIS, start=q.START_HOT, seed=4321,
-                                   n_therm=200, n_meas=args.fp32_meas, epsilon=0.5, hits=4))
-                 for bc in (q.BC_PCC, q.BC_PMC)}
+        eng = q.Engine(precision=prec, max_jobs=2 * len(seeds), budget_bytes=2 << 30, use_graph=True)
+        slots = {(bc, sd): eng.add(q.Job(32, 32, 48, 4.0, bc=bc, algo=q.ALGO_METROPOLIS, start=q.START_HOT,
+                                         seed=sd, n_therm=200, n_meas=args.fp32_meas, epsilon=0.5, hits=4))
+                 for bc in (q.BC_PCC, q.BC_PMC) for sd in seeds}
         eng.run()
-        for bc, s in slots.items():
-            runs[(prec, bc)] = eng.read(s)
+        for key, s in slots.items():
+            runs[(prec,) + key] = eng.read(s)
         eng.close()
     for bc, name in ((q.BC_PCC, "pcc"), (q.BC_PMC, "pmc")):
-        P64, P32 = runs[(0, bc)]["P"], runs[(1, bc)]["P"]
-        m64, e64 = q.stats.binned_error(P64, 20)
-        m32, e32 = q.stats.binned_error(P32, 20)
-        sig = float(np.hypot(e64, e32))
-        out[name] = {"P_fp64": m64, "err_fp64": e64, "P_fp32": m32, "err_fp32": e32,
-                     "diff": m32 - m64, "diff_over_sigma": (m32 - m64) / sig if sig else 0.0,
-                     "accept_fp64": runs[(0, bc)]["accept"], "accept_fp32": runs[(1, bc)]["accept"]}
+        rows = []
+        for sd in seeds:
+            P64, P32 = runs[(0, bc, sd)]["P"], runs[(1, bc, sd)]["P"]
+            m64, e64 = q.stats.binned_error(P64, 20)
+            m32, e32 = q.stats.binned_error(P32, 20)
+            sig = float(np.hypot(e64, e32))
+            rows.append({"seed": sd, "P_fp64": m64, "err_fp64": e64, "P_fp32": m32, "err_fp32": e32,
+                         "diff": m32 - m64, "diff_over_sigma": (m32 - m64) / sig if sig else 0.0,
+                         "accept_fp64": runs[(0, bc, sd)]["accept"], "accept_fp32": runs[(1, bc, sd)]["accept"]})
+        out[name] = dict(rows[0])
+        if len(rows) > 1:
+            z = np.array([r["diff_over_sigma"] for r in rows])
+            d = np.array([r["diff"] for r in rows])
+            sg = np.array([np.hypot(r["err_fp64"], r["err_fp32"]) for r in rows])
+            md, me = float(d.mean()), float(np.sqrt(np.sum(sg ** 2)) / len(rows))
+            # control: two FP64 runs of different seeds, the spread of the
+            # statistic when only the chain differs
+            ctl = []
+            for a, b in zip(rows[0::2], rows[1::2]):
+                sg2 = np.hypot(a["err_fp64"], b["err_fp64"])
+                ctl.append(float((b["P_fp64"] - a["P_fp64"]) / sg2) if sg2 else 0.0)
+            out[name]["seeds"] = {"n": len(rows), "diff_over_sigma": [float(x) for x in z],
+                                  "within_1_sigma": int(np.sum(np.abs(z) <= 1.0)),
+                                  "within_2_sigma": int(np.sum(np.abs(z) <= 2.0)),
+                                  "chi2": float(np.sum(z ** 2)),
+                                  "mean_diff": md, "mean_diff_err": me,
+                                  "mean_diff_over_err": md / me if me else 0.0,
+                                  "control_fp64_seed_pairs_diff_over_sigma": ctl}
     return out
 
 
