@@ -153,6 +153,15 @@ def test_strip_and_slab_block_orders(gpu, oracle_mod, shape, bc):
     run_pair(gpu, oracle_mod, "strips", *shape, 4.5, bc, 1, 0, 37, 2, updates=(0, 1))
 
 
+@pytest.mark.parametrize("bc", [1, 2])
+def test_beyond_the_infinity_cache_pcc_pmc(gpu, oracle_mod, bc):
+    """128x128x160 FP64 (283 MB of links and RNG state, more than the 256 MB
+    Infinity Cache): the heat-bath launches stream their RNG state with
+    nontemporal hints; PCC and PMC, one heat-bath + over-relaxation step."""
+    run_pair(gpu, oracle_mod, f"stream-rng bc {bc}", 128, 128, 160, 4.5, bc=bc, algo=1, start=0, seed=43,
+             nsweeps=1, updates=(0, 1))
+
+
 @pytest.mark.parametrize("start", [1, 2])
 def test_cold_and_haar_starts(gpu, oracle_mod, start):
     for bc in (0, 1, 2):
