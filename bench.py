@@ -6,7 +6,9 @@ periodic, heat-bath followed by one over-relaxation sweep, beta 5.0, hot
 start with Haar-random links from seed 7, FP64 links.  One step = one
 heat-bath sweep (6 launches) + one over-relaxation sweep (6 launches) + the
 fused energy/plaquette measurement, with the reunitarisation every 100
-steps, as in the reference's measurement loop (kernel.cu:192-208).  Before
+steps, as in the reference's measurement loop (kernel.cu:192-208); as in the
+C driver, each measurement is carried by the next step's first launch
+(su2_mc_up_measured) and the last one runs on its own, inside the timing.  Before
 anything is timed the lattice is thermalised with the configuration's 200
 heat-bath + over-relaxation sweeps (--therm, the thermalisation loop of
 kernel.cu:161-179); --warmup steps after that are a timing warm-up only.
@@ -111,13 +113,29 @@ def gpu_bench(args):
     res = q.DeviceBuffer(8 * 2 * max(1, args.steps))
     res.zero(stream)
 
+    # the measurement of a step is held back and carried by the next step's
+    # first launch (su2_mc_up_measured, the same bits as en_av before the
+    # sweep), as the C driver does; flush() measures the last one on its own
+    pending = [None]
+
     def step(i, res_slot=None):
-        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
+        if pending[0] is not None:
+            s = pending[0]
+            g.mc_up_measured(q.BC_PERIODIC, res.value + 16 * s, res.value + 16 * s + 8, q.ALGO_HEATBATH,
+                             q.UPDATE_LINK)
+            pending[0] = None
+        else:
+            g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
         g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
         if i % 100 == 0:
             g.normalize()
-        if res_slot is not None:
-            g.en_av_into(False, res.value + 16 * res_slot, res.value + 16 * res_slot + 8)
+        pending[0] = res_slot
+
+    def flush():
+        if pending[0] is not None:
+            s = pending[0]
+            g.en_av_into(False, res.value + 16 * s, res.value + 16 * s + 8)
+            pending[0] = None
 
     # thermalisation (kernel.cu:161-179): reunitarise before the sweep every
     # 100 sweeps and once after the loop
@@ -129,6 +147,7 @@ def gpu_bench(args):
     g.normalize()
     for i in range(1, args.warmup + 1):
         step(i, 0)
+    flush()
     g.synchronize()
     res.zero(stream)
     g.synchronize()
@@ -140,6 +159,7 @@ def gpu_bench(args):
     L.check(lib.su2_event_record(ev0, stream), "record")
     for i in range(1, args.steps + 1):
         step(i, i - 1)
+    flush()
     L.check(lib.su2_event_record(ev1, stream), "record")
     ms = C.c_float()
     L.check(lib.su2_event_elapsed_ms(ev0, ev1, C.byref(ms)), "elapsed")

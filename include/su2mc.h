@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define SU2MC_ABI_VERSION 3
+#define SU2MC_ABI_VERSION 4
 
 /* ---- status codes ---- */
 #define SU2_OK 0
@@ -171,11 +171,25 @@ int su2_normalize(const su2_lattice *l, void *phi, int precision, void *stream);
 int su2_en_av(const su2_lattice *l, const void *phi, int precision, int pmc, double *res, double *res2, void *work,
               void *stream);
 
+/* en_av of the field as it is (kernel.cu:208), then one sweep
+ * (kernel.cu:192-202, mc_up_per / mc_up_pcc / mc_up_pmc by bc): the same
+ * results, bit for bit, as su2_en_av(l, phi, precision, bc == SU2_BC_PMC,
+ * res, res2, work, stream) followed by the sweep, in fewer passes over the
+ * field.  The reference's run loop measures after the last sweep of a
+ * measurement and sweeps again right after; a driver that holds a
+ * measurement back until its next sweep makes the pair one call.  The
+ * measurement rides on the sweep's first launch (y, parity 0), which reads
+ * nearly every link the measurement needs before it writes; with a sweep
+ * block size other than 256 threads (su2_set_sweep_block) the two run
+ * separately.  work as for su2_en_av. */
+int su2_mc_up_measured(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int bc,
+                       int algo, int update, double *res, double *res2, void *work, void *stream);
+
 /* No counterpart in the reference: su2_en_av for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  The same kernels and summation
  * order as su2_en_av, so the results are identical bit for bit, with the
- * block partial sums in the library's own static device array (4 MiB per
+ * block partial sums in the library's own static device array (2 MiB per
  * device, held until the process ends) instead of a work buffer; no
  * allocation of any kind.  Lattices of at most SU2_EN_AV_NOALLOC_MAX_SITES
  * sites, else SU2_EINVAL.  All calls on one device share that array: calls

@@ -99,6 +99,7 @@ PROTOTYPES = {
     "su2_mc_up_pcc": (_I, [_L, _P, _I, _P, _U32P, _I, _I, _P]),
     "su2_mc_up_pmc": (_I, [_L, _P, _I, _P, _U32P, _I, _I, _P]),
     "su2_mc_update": (_I, [_L, _P, _I, _P, _U32P, _I, _I, _I, _I, _I, _P]),
+    "su2_mc_up_measured": (_I, [_L, _P, _I, _P, _U32P, _I, _I, _I, _P, _P, _P, _P]),
     "su2_normalize": (_I, [_L, _P, _I, _P]),
     "su2_en_av": (_I, [_L, _P, _I, _I, _P, _P, _P, _P]),
     "su2_en_av_noalloc": (_I, [_L, _P, _I, _I, _P, _P, _P]),
@@ -175,7 +176,7 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.su2_abi_version() != 3:
+    if lib.su2_abi_version() != 4:
         raise RuntimeError("libsu2mc ABI version mismatch")
     _lib = lib
     return lib

@@ -29,6 +29,7 @@
  */
 #include "su2_batch.h"
 #include "su2_observe.h"
+#include "su2_sweep_site.h"
 
 namespace {
 
@@ -194,29 +195,9 @@ __global__ __launch_bounds__(256) void k_bnormalize(const su2_bhead *__restrict_
 }
 
 /* en_av pass 1 for the lattices that measure this tick; logical block b of
- * a lattice has the single-lattice layout (p = b & 1, chunk b >> 1). */
-template <int PMC, typename LinkT>
-__device__ static inline void en_av_block(const su2_bdesc &d, uint32_t b)
-{
-    const uint32_t p = b & 1u;
-    const uint32_t h = (b >> 1) * SU2_EN_BLOCK + threadIdx.x;
-    double e = 0.0, pl = 0.0;
-    if (h < d.g.Vh) {
-        const uint32_t q = su2_div(h, d.g.div_lyh);
-        const uint32_t yh = h - q * d.g.Lyh;
-        const uint32_t t = su2_div(q, d.g.div_lx);
-        const uint32_t x = q - t * d.g.Lx;
-        su2_site_plaquettes<PMC, LinkT>((const LinkT *)d.U, d.g, p, h, t, x, yh, &e, &pl);
-    }
-    su2_block_sum2(e, pl, SU2_EN_BLOCK);
-    if (threadIdx.x == 0) {
-        d.part[2 * b] = e;
-        d.part[2 * b + 1] = pl;
-    }
-}
-
+ * a lattice has the single-lattice layout and bits (su2_en_av_owner_block). */
 template <typename LinkT>
-__global__ __launch_bounds__(256) void k_ben_av_partial(const su2_bhead *__restrict__ H,
+__global__ __launch_bounds__(2 * SU2_EN_BLOCK) void k_ben_av_partial(const su2_bhead *__restrict__ H,
                                                         const su2_bdesc *__restrict__ D,
                                                         const uint32_t *__restrict__ B, uint32_t max_desc,
                                                         const uint32_t *__restrict__ ctr)
@@ -231,11 +212,7 @@ __global__ __launch_bounds__(256) void k_ben_av_partial(const su2_bhead *__restr
     for (uint32_t vb = vb0; vb < vend; vb += step) {
         const su2_bdesc &d = D[find_desc(Bk, s_begin, lds, n, vb)];
         if (!measure_now(d, ctr[d.slot])) continue;
-        const uint32_t b = vb - d.en_begin;
-        if (d.bc == SU2_K_BC_PMC)
-            en_av_block<1, LinkT>(d, b);
-        else
-            en_av_block<0, LinkT>(d, b);
+        su2_en_av_owner_block<LinkT>((const LinkT *)d.U, d.g, d.bc == SU2_K_BC_PMC, vb - d.en_begin, d.part);
         __syncthreads(); /* su2_block_sum2's shared array is reused by the next block */
     }
 }
@@ -336,9 +313,9 @@ hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, co
                              uint32_t *ctr, uint32_t grid, uint32_t final_grid, hipStream_t st)
 {
     if (f32)
-        hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(SU2_EN_BLOCK), 0, st, H, D, B, max_desc, ctr);
+        hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(su2_en_av_threads(1)), 0, st, H, D, B, max_desc, ctr);
     else
-        hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(SU2_EN_BLOCK), 0, st, H, D, B, max_desc, ctr);
+        hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(su2_en_av_threads(0)), 0, st, H, D, B, max_desc, ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_ben_av_final, dim3(final_grid), dim3(SU2_EN_FINAL_BLOCK), 0, st, H, D, ctr);
@@ -351,7 +328,4 @@ hipError_t su2_launch_btick_end(const su2_bhead *H, const su2_bdesc *D, uint32_t
     return hipGetLastError();
 }
 
-uint32_t su2_en_av_logical_blocks(const su2_kgeom &g)
-{
-    return su2_grid_blocks(2ull * ((g.Vh + SU2_EN_BLOCK - 1) / SU2_EN_BLOCK) * SU2_EN_BLOCK, SU2_EN_BLOCK);
-}
+uint32_t su2_en_av_logical_blocks(const su2_kgeom &g) { return su2_en_av_blocks(g); }

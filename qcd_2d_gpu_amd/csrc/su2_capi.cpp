@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "su2_internal.h"
+#include "su2_observe.h"
 #include "su2mc.h"
 
 namespace {
@@ -84,8 +85,12 @@ int check_couplings(const su2_lattice *l, int algo, int update)
     return SU2_OK;
 }
 
+/* One sweep; with res non-null it also measures the field it starts from
+ * (en_av into res, res2 with work), on its first launch when the sweep block
+ * is SU2_EN_BLOCK threads, else with su2_launch_en_av before it (the same
+ * partials, so the same bits either way). */
 int sweep(const su2_lattice *l, void *phi, int prec, int *accept, uint32_t *states, int bc, int algo, int update,
-          void *stream)
+          void *stream, double *res = nullptr, double *res2 = nullptr, void *work = nullptr)
 {
     int rc = check_lattice(l);
     if (rc) return rc;
@@ -107,13 +112,26 @@ int sweep(const su2_lattice *l, void *phi, int prec, int *accept, uint32_t *stat
      * Heat-bath and over-relaxation accept every update they make, so the
      * t launches add each half-site's count for the whole sweep and the y
      * and x launches count nothing; Metropolis counts in every launch. */
+    const int block = g_sweep_block;
+    double *mw = nullptr;
+    if (res) {
+        if (block == (int)SU2_EN_BLOCK) {
+            mw = (double *)work;
+        } else {
+            hipError_t e = su2_launch_en_av(bc == SU2_BC_PMC, prec, phi, g, res, res2, (double *)work, S(stream));
+            if (e != hipSuccess) return hip_fail(e, "en_av launch");
+        }
+    }
     for (int mu = 0; mu < 3; ++mu)
         for (int par = 0; par < 2; ++par) {
             const int acc_mode = upd == SU2_K_METRO ? SU2_K_ACC_LAUNCH
                                                     : (mu == SU2_DIR_T ? SU2_K_ACC_SWEEP : SU2_K_ACC_NONE);
+            double *m = (mu == SU2_DIR_Y && par == 0) ? mw : nullptr;
             hipError_t e =
-                su2_launch_sweep(mu, bc, upd, prec, phi, states, accept, g, p, par, g_sweep_block, acc_mode, S(stream));
+                su2_launch_sweep(mu, bc, upd, prec, phi, states, accept, g, p, par, block, acc_mode, S(stream), m);
             if (e != hipSuccess) return hip_fail(e, "sweep launch");
+            if (m && (e = su2_launch_en_av_final(g, m, res, res2, S(stream))) != hipSuccess)
+                return hip_fail(e, "en_av launch");
         }
     return SU2_OK;
 }
@@ -251,6 +269,13 @@ int su2_mc_up_pmc(const su2_lattice *l, void *phi, int precision, int *accept, u
                   int update, void *stream)
 {
     return sweep(l, phi, precision, accept, states, SU2_BC_PMC, algo, update, stream);
+}
+
+int su2_mc_up_measured(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int bc,
+                       int algo, int update, double *res, double *res2, void *work, void *stream)
+{
+    if (!res || !res2 || !work) return fail(SU2_EINVAL, "NULL pointer passed to mc_up_measured");
+    return sweep(l, phi, precision, accept, states, bc, algo, update, stream, res, res2, work);
 }
 
 int su2_mc_update(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int bc, int algo,

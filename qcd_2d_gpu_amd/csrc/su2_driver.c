@@ -333,10 +333,29 @@ static int sweep_kind(run_ctx *r, int update)
     }
 }
 
-/* one update sweep, then the over-relaxation sweeps (extension) */
-static int one_sweep(run_ctx *r)
+/* en_av of measurement i (kernel.cu:208): into slots end[i-1], end2[i-1] */
+static int measure(run_ctx *r, int i)
 {
-    int rc = sweep_kind(r, SU2_UPDATE_LINK);
+    return su2_en_av(&r->l, r->phi, r->prec, r->c->boundary == SU2_BC_PMC, (double *)r->end + i - 1,
+                     (double *)r->end2 + i - 1, r->work, r->stream);
+}
+
+/* one update sweep, then the over-relaxation sweeps (extension).  With
+ * *pending > 0, the en_av of measurement *pending, held back since the end
+ * of its sweeps, runs first, on the update sweep's first launch
+ * (su2_mc_up_measured: the same bits as su2_en_av before the sweep). */
+static int one_sweep(run_ctx *r, int *pending)
+{
+    int rc;
+    if (*pending > 0) {
+        const int i = *pending;
+        rc = su2_mc_up_measured(&r->l, r->phi, r->prec, (int *)r->acc, (uint32_t *)r->st, r->c->boundary,
+                                r->c->algo, SU2_UPDATE_LINK, (double *)r->end + i - 1, (double *)r->end2 + i - 1,
+                                r->work, r->stream);
+        *pending = 0;
+    } else {
+        rc = sweep_kind(r, SU2_UPDATE_LINK);
+    }
     if (rc) return rc;
     r->updates += 3ll * r->l.V;
     for (int k = 0; k < r->c->or_sweeps; ++k) {
@@ -473,10 +492,11 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
     const int block_t = c->Nblock_term > 0 ? c->N_term / c->Nblock_term : 0;
     const int block_m = c->N_block > 0 ? N / c->N_block : 0;
     double t0 = now_s(), t_n = t0;
+    int pending = 0; /* a measurement whose en_av waits for the next sweep */
     if (c->readconf < 2) { /* kernel.cu:161-187 */
         for (int i = 1; i < c->N_term + 1; ++i) {
             if (i % c->normalize_every == 0) CK(su2_normalize(l, r.phi, prec, stream));
-            CK(one_sweep(&r));
+            CK(one_sweep(&r, &pending));
             if (c->verbose && block_t > 0 && i % block_t == 0) {
                 CK(su2_stream_synchronize(stream));
                 const double t = now_s();
@@ -505,10 +525,15 @@ int su2_run(const su2_run_config *c, su2_run_result *res)
     r.per_meas = (long long)c->N_cor * l->d * (hits + c->or_sweeps);
     const int block_s = N / c->Nsave > 0 ? N / c->Nsave : N;
     for (int i = 1; i < N + 1; ++i) { /* kernel.cu:192-236 */
-        for (int j = 0; j < c->N_cor; ++j) CK(one_sweep(&r));
+        for (int j = 0; j < c->N_cor; ++j) CK(one_sweep(&r, &pending));
         if (i % c->normalize_every == 0) CK(su2_normalize(l, r.phi, prec, stream));
-        CK(su2_en_av(l, r.phi, prec, c->boundary == SU2_BC_PMC, (double *)r.end + i - 1, (double *)r.end2 + i - 1,
-                     r.work, stream));
+        /* en_av of measurement i (kernel.cu:208) on the next sweep's first
+         * launch, or here when the series is read before that */
+        pending = i;
+        if (i == N || (i % block_s == 0 && i != N)) {
+            CK(measure(&r, i));
+            pending = 0;
+        }
         if (c->verbose && block_m > 0 && i % block_m == 0) {
             CK(su2_stream_synchronize(stream));
             const double t = now_s();

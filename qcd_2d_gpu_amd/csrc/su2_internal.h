@@ -144,9 +144,12 @@ static inline uint32_t su2_grid_blocks(uint64_t nthreads, uint32_t block)
 }
 
 /* ---- launchers (host), defined in the .hip files ---- */
+/* meas_work non-null: the launch (mu = y, parity 0, block = SU2_EN_BLOCK
+ * threads, otherwise hipErrorInvalidValue) also writes the en_av partials of
+ * the field it starts from there; su2_launch_en_av_final completes them. */
 hipError_t su2_launch_sweep(int mu, int bc, int upd, int f32, void *U, uint32_t *rng, int *accept,
                             const su2_kgeom &g, const su2_kphys &p, int parity, int block, int acc_mode,
-                            hipStream_t st);
+                            hipStream_t st, double *meas_work = nullptr);
 hipError_t su2_launch_setup_rng(uint32_t *rng, const su2_kgeom &g, uint64_t seed, hipStream_t st);
 /* start: 0 reference hot start, 1 cold, 2 reference PCC hot start, 3 Haar,
  * 4 Haar with PCC identity links at y = 0 */
@@ -155,6 +158,8 @@ hipError_t su2_launch_normalize(int f32, void *U, const su2_kgeom &g, hipStream_
 hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g, double *res, double *res2,
                             double *work, hipStream_t st);
 size_t su2_en_av_work_doubles(const su2_kgeom &g);
+/* pass 2 of en_av over partials written by a measuring sweep launch */
+hipError_t su2_launch_en_av_final(const su2_kgeom &g, const double *work, double *res, double *res2, hipStream_t st);
 /* en_av with its partial sums in the library's own static device array (no
  * allocation); lattices of at most SU2_EN_AV_NOALLOC_MAX_SITES sites (=
  * SU2_EN_AV_NOALLOC_MAX_SITES of su2mc.h). */

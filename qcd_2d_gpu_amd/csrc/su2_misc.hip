@@ -6,6 +6,7 @@
 #include <mutex>
 
 #include "su2_observe.h"
+#include "su2_sweep_site.h"
 
 namespace {
 
@@ -75,31 +76,18 @@ template <typename LinkT> __global__ __launch_bounds__(256) void k_normalize(Lin
         su2_link_io<LinkT>::store(U + i, su2_q_normalize(su2_link_io<LinkT>::load(U + i)));
 }
 
-/* Pass 1 of en_av, one site per thread.  Logical block b covers the sites
- * (p = b & 1, h in chunk b >> 1): the two parities of a chunk of 256
- * half-sites are adjacent blocks, which read each other's neighbourhoods,
- * and the XCD-aware remap keeps runs of chunks on one XCD's L2.  Block sums
- * go to work[2 b]. */
+/* Pass 1 of en_av in owner form (su2_observe.h): logical block b owns
+ * parity-0 half-sites [b * SU2_EN_BLOCK, ...) and writes partials 2b,
+ * 2b + 1 (su2_en_av_owner_block).  The blocks run in the sweeps' XCD-aware
+ * order (su2_block_of: the t-neighbour rows a block reads stay in its XCD's
+ * L2). */
 template <int PMC, typename LinkT>
-__global__ __launch_bounds__(256) void k_en_av_partial(const LinkT *__restrict__ U, su2_kgeom g,
-                                                       double *__restrict__ work)
+__global__ __launch_bounds__(2 * SU2_EN_BLOCK) void k_en_av_partial(const LinkT *__restrict__ U, su2_kgeom g,
+                                                                    double *__restrict__ work, uint32_t order,
+                                                                    uint32_t strip, uint32_t bpp)
 {
-    const uint32_t b = su2_xcd_block();
-    const uint32_t p = b & 1u;
-    const uint32_t h = (b >> 1) * SU2_EN_BLOCK + threadIdx.x;
-    double e = 0.0, pl = 0.0;
-    if (h < g.Vh) {
-        const uint32_t q = su2_div(h, g.div_lyh);
-        const uint32_t yh = h - q * g.Lyh;
-        const uint32_t t = su2_div(q, g.div_lx);
-        const uint32_t x = q - t * g.Lx;
-        su2_site_plaquettes<PMC, LinkT>(U, g, p, h, t, x, yh, &e, &pl);
-    }
-    su2_block_sum2(e, pl, SU2_EN_BLOCK);
-    if (threadIdx.x == 0) {
-        work[2 * b] = e;
-        work[2 * b + 1] = pl;
-    }
+    const uint32_t b = su2_block_of(order, strip, bpp);
+    su2_en_av_owner_block<LinkT>(U, g, PMC, b, work);
 }
 
 /* Pass 2: one block of 1024 threads adds the partials in a fixed order
@@ -211,11 +199,11 @@ __global__ __launch_bounds__(256) void k_stream_probe(su2_v4u *__restrict__ dst,
 
 /* Partial sums of su2_en_av_noalloc: two doubles per pass-1 block of a
  * lattice of at most SU2_EN_AV_NOALLOC_MAX_SITES sites (2^25 half-sites,
- * 2^17 chunks of 256, two blocks each): 4 MiB, held by each device the
- * library runs on until the process ends.  Every call on a device writes
+ * 2^17 blocks of 256): 2 MiB, held by each device the library runs on until
+ * the process ends.  Every call on a device writes
  * every partial it reads, so nothing in it needs clearing; calls that
  * overlap on one device (other streams, other host threads) race on it. */
-#define SU2_EN_AV_SCRATCH_DOUBLES ((size_t)2 * 2 * (size_t)(SU2_EN_AV_NOALLOC_MAX_SITES / 2 / SU2_EN_BLOCK))
+#define SU2_EN_AV_SCRATCH_DOUBLES ((size_t)2 * (size_t)(SU2_EN_AV_NOALLOC_MAX_SITES / 2 / SU2_EN_BLOCK))
 static __device__ __align__(256) double scratch[SU2_EN_AV_SCRATCH_DOUBLES];
 
 /* The device address of `scratch`, looked up once per device with that
@@ -315,36 +303,38 @@ hipError_t su2_launch_normalize(int f32, void *U, const su2_kgeom &g, hipStream_
     return hipGetLastError();
 }
 
-/* en_av launch shape: one site per thread, two blocks (the two parities)
- * per chunk of SU2_EN_BLOCK half-sites. */
-static uint32_t en_av_blocks(const su2_kgeom &g)
-{
-    return su2_grid_blocks(2ull * ((g.Vh + SU2_EN_BLOCK - 1) / SU2_EN_BLOCK) * SU2_EN_BLOCK, SU2_EN_BLOCK);
-}
+/* partial sums, 2 per logical block */
+size_t su2_en_av_work_doubles(const su2_kgeom &g) { return 2 * (size_t)su2_en_av_blocks(g); }
 
-/* partial sums, 2 per block */
-size_t su2_en_av_work_doubles(const su2_kgeom &g) { return 2 * (size_t)en_av_blocks(g); }
+hipError_t su2_launch_en_av_final(const su2_kgeom &g, const double *work, double *res, double *res2, hipStream_t st)
+{
+    const double invV = 1.0 / (2.0 * (double)g.Vh);
+    hipLaunchKernelGGL(k_en_av_final, dim3(1), dim3(SU2_EN_FINAL_BLOCK), 0, st, work, su2_en_av_blocks(g), invV, res,
+                       res2);
+    return hipGetLastError();
+}
 
 hipError_t su2_launch_en_av(int pmc, int f32, const void *U, const su2_kgeom &g, double *res, double *res2,
                             double *work, hipStream_t st)
 {
-    const uint32_t nb = en_av_blocks(g);
+    const uint32_t nb = su2_en_av_blocks(g);
+    uint32_t order, strip, bpp;
+    su2_pick_block_order(g, SU2_EN_BLOCK, !f32, &order, &strip, &bpp);
+    const dim3 G(nb), B(su2_en_av_threads(f32));
     if (f32) {
         if (pmc)
-            hipLaunchKernelGGL((k_en_av_partial<1, su2_l32>), dim3(nb), dim3(SU2_EN_BLOCK), 0, st, (const su2_l32 *)U, g, work);
+            hipLaunchKernelGGL((k_en_av_partial<1, su2_l32>), G, B, 0, st, (const su2_l32 *)U, g, work, order, strip, bpp);
         else
-            hipLaunchKernelGGL((k_en_av_partial<0, su2_l32>), dim3(nb), dim3(SU2_EN_BLOCK), 0, st, (const su2_l32 *)U, g, work);
+            hipLaunchKernelGGL((k_en_av_partial<0, su2_l32>), G, B, 0, st, (const su2_l32 *)U, g, work, order, strip, bpp);
     } else {
         if (pmc)
-            hipLaunchKernelGGL((k_en_av_partial<1, su2_l64>), dim3(nb), dim3(SU2_EN_BLOCK), 0, st, (const su2_l64 *)U, g, work);
+            hipLaunchKernelGGL((k_en_av_partial<1, su2_l64>), G, B, 0, st, (const su2_l64 *)U, g, work, order, strip, bpp);
         else
-            hipLaunchKernelGGL((k_en_av_partial<0, su2_l64>), dim3(nb), dim3(SU2_EN_BLOCK), 0, st, (const su2_l64 *)U, g, work);
+            hipLaunchKernelGGL((k_en_av_partial<0, su2_l64>), G, B, 0, st, (const su2_l64 *)U, g, work, order, strip, bpp);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const double invV = 1.0 / (2.0 * (double)g.Vh);
-    hipLaunchKernelGGL(k_en_av_final, dim3(1), dim3(SU2_EN_FINAL_BLOCK), 0, st, (const double *)work, nb, invV, res, res2);
-    return hipGetLastError();
+    return su2_launch_en_av_final(g, work, res, res2, st);
 }
 
 hipError_t su2_launch_en_av_noalloc(int pmc, int f32, const void *U, const su2_kgeom &g, double *res, double *res2,

@@ -1,65 +1,71 @@
 /*
- * su2_observe.h - the per-site plaquette terms of en_av and the
- * deterministic block sum, shared by the single-lattice and the batched
- * reductions (so a lattice's energy and plaquette come out bit for bit the
- * same in both).
+ * su2_observe.h - the plaquette terms of en_av and the deterministic block
+ * sums, shared by the single-lattice en_av, the sweep launch that carries a
+ * measurement, and the batched reduction (so a lattice's energy and
+ * plaquette come out bit for bit the same in all of them).
  */
 #ifndef SU2_OBSERVE_H
 #define SU2_OBSERVE_H
 
-#include "su2_sweep_site.h"
+#include "su2_internal.h"
+#include "su2_quat.h"
 
-/* The three plaquettes at site n of parity p (functions/observables.h:1-21),
- * each returned as 1 - 1/2 tr P:
- *   P_yx = U_y(n) U_x(n+y) U_y(n+x)^dag U_x(n)^dag
- *   P_yt = U_y(n) U_t(n+y) U_y(n+t)^dag U_t(n)^dag
- *   P_xt = U_x(n) U_t(n+x) U_x(n+t)^dag U_t(n)^dag
- * The reduction of each site (functions/observables.h:99-141) gives
- * E_site = -(1 - P_xt) and P_site = (3 - a - b - c) / 3; with PMC the y-x and
- * y-t plaquettes of y = Ly-1 count as 1 (a = b = 0).  (t, x, y') is the
- * decoded half-site shared by both parities. */
-template <int PMC, typename LinkT>
-__device__ static inline void su2_site_plaquettes(const LinkT *__restrict__ U, const su2_kgeom &g, uint32_t p, uint32_t h,
-                                              uint32_t t, uint32_t x, uint32_t yh, double *e, double *pl)
+/*
+ * en_av (functions/observables.h:1-21, 99-141) in owner form.  With
+ *   a = 1 - 1/2 tr P_yx,  b = 1 - 1/2 tr P_yt,  c = 1 - 1/2 tr P_xt,
+ *   P_yx(n) = U_y(n) U_x(n+y) U_y(n+x)^dag U_x(n)^dag  (P_yt, P_xt alike),
+ * the reference sums E_site = -c and P_site = (3 - a - b - c) / 3 over the
+ * sites (with PMC, a = b = 0 at y = Ly-1).  Here the thread of parity-0
+ * half-site h, site m, owns six plaquettes:
+ *   a(m), b(m), c(m), a(m - x), c(m - x), b(m - t),
+ * and contributes e = -(c(m) + c(m - x)) and
+ * p = (6 - a(m) - b(m) - c(m) - a(m - x) - c(m - x) - b(m - t)) / 3.
+ * m -> m - x and m -> m - t map the parity-0 sites one to one onto the
+ * parity-1 sites (Lx and Lt are even), so every site's three plaquettes are
+ * counted once, and the sums equal the reference's up to rounding.  m, m - x
+ * and m - t share y, so the PMC rule is one test per thread.  The links
+ * these six plaquettes read are the twelve of the U_y staple at m, U_y(m)
+ * itself and four more (U_t(m+x), U_x(m+t), U_x(m+t-x), U_t(m-x)); of them
+ * only U_y(m) is written by the sweep launch (y, parity 0), and only by this
+ * thread, so that launch can carry the measurement of the field it starts
+ * from (su2_site_staple).
+ */
+struct su2_owner_plaq {
+    double a, b, c, a1, c1, b2; /* a(m) b(m) c(m) a(m-x) c(m-x) b(m-t) */
+    int edge;                   /* y(m) = Ly - 1 */
+};
+
+/* 1 - 1/2 tr (A B C^dag D^dag) */
+__host__ __device__ static inline double su2_plaq_term(su2_q A, su2_q B, su2_q C, su2_q D)
 {
-    const size_t Vh = g.Vh;
-    const uint32_t y = 2u * yh + (((t + x) & 1u) ^ p);
-    const uint32_t yp = (y + 1u == g.Ly) ? 0u : y + 1u;
-    const uint32_t xp = (x + 1u == g.Lx) ? 0u : x + 1u;
-    const uint32_t tp = (t + 1u == g.Lt) ? 0u : t + 1u;
-    const uint32_t h_py = (yp >> 1) + x * g.Lyh + t * g.plane;
-    const uint32_t h_px = yh + xp * g.Lyh + t * g.plane;
-    const uint32_t h_pt = yh + x * g.Lyh + tp * g.plane;
-    const LinkT *Us = U + (size_t)(p * 3) * Vh;
-    const LinkT *Uo = U + (size_t)((1u - p) * 3) * Vh;
-
-    const su2_q uy = su2_link_io<LinkT>::load(Us + 0 * Vh + h);
-    const su2_q ux = su2_link_io<LinkT>::load(Us + 1 * Vh + h);
-    const su2_q ut = su2_link_io<LinkT>::load(Us + 2 * Vh + h);
-    const su2_q ux_py = su2_link_io<LinkT>::load(Uo + 1 * Vh + h_py);
-    const su2_q ut_py = su2_link_io<LinkT>::load(Uo + 2 * Vh + h_py);
-    const su2_q uy_px = su2_link_io<LinkT>::load(Uo + 0 * Vh + h_px);
-    const su2_q ut_px = su2_link_io<LinkT>::load(Uo + 2 * Vh + h_px);
-    const su2_q uy_pt = su2_link_io<LinkT>::load(Uo + 0 * Vh + h_pt);
-    const su2_q ux_pt = su2_link_io<LinkT>::load(Uo + 1 * Vh + h_pt);
-
-    double a = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(uy, ux_py), uy_px), ux);
-    double b = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(uy, ut_py), uy_pt), ut);
-    double c = 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(ux, ut_px), ux_pt), ut);
-    if (PMC && y == g.Ly - 1u) {
-        a = 0.0;
-        b = 0.0;
-    }
-    *e = -c;
-    *pl = (3.0 - a - b - c) / 3.0;
+    return 1.0 - su2_q_mul_nd_re(su2_q_mul_nd(su2_q_mul(A, B), C), D);
 }
 
-/* Workgroup size of every en_av pass-1 launch (single-lattice and batched):
- * logical block b covers half-sites [(b >> 1) * SU2_EN_BLOCK, ...) of parity
- * b & 1, which fixes the partials and their summation order. */
+/* The thread's contribution to the energy and plaquette sums. */
+__host__ __device__ static inline void su2_owner_finish(su2_owner_plaq q, int pmc, double *e, double *pl)
+{
+    if (pmc && q.edge) {
+        q.a = 0.0;
+        q.b = 0.0;
+        q.a1 = 0.0;
+        q.b2 = 0.0;
+    }
+    *e = -(q.c + q.c1);
+    *pl = (6.0 - q.a - q.b - q.c - q.a1 - q.c1 - q.b2) / 3.0;
+}
+
+/* Workgroup size of every launch that makes en_av partials (single-lattice,
+ * the measuring sweep launch, batched): logical block b, the owners of
+ * parity-0 half-sites [b * SU2_EN_BLOCK, (b + 1) * SU2_EN_BLOCK), writes the
+ * partial pair 2b, 2b + 1, which fixes the partials and their summation
+ * order.  Blocks past the lattice write zeros. */
 #define SU2_EN_BLOCK 256u
 /* Workgroup size of every en_av pass-2 launch. */
 #define SU2_EN_FINAL_BLOCK 1024u
+
+/* Logical blocks (partial pairs) of a lattice's en_av: the half-sites in
+ * blocks of SU2_EN_BLOCK, padded to a multiple of 8 (the XCD count). */
+static inline uint32_t su2_en_av_blocks(const su2_kgeom &g) { return su2_grid_blocks(g.Vh, SU2_EN_BLOCK); }
 
 /* Deterministic block sum of two doubles over a workgroup of nthreads (the
  * launch's blockDim.x, a multiple of 64 up to 1024): wavefront shuffles in a

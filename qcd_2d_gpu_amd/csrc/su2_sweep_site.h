@@ -22,6 +22,7 @@
 #define SU2_SWEEP_SITE_H
 
 #include "su2_internal.h"
+#include "su2_observe.h"
 #include "su2_update.h"
 
 template <typename LinkT> struct su2_link_io;
@@ -65,6 +66,79 @@ template <> struct su2_link_io<su2_l32> {
     }
 };
 
+/* Workgroup -> block of 256 half-sites.  The hardware deals workgroups
+ * round-robin to the 8 XCDs, each with its own 4 MB L2; the block order
+ * decides which part of the lattice an XCD walks, and in which order.
+ *
+ * ORDER_SLAB: XCD k takes the k-th contiguous eighth of the lattice (a slab
+ * of t-planes) and walks it in memory order.
+ *
+ * ORDER_STRIPS (the host sets it only when every plane is a whole number
+ * of blocks, the slabs are whole planes and the plane's bpp blocks split
+ * into strips of `strip` blocks): the XCD walks its slab strip by strip,
+ * all its t-planes for one strip of x-rows before the next, so the
+ * t-neighbour rows of a strip are re-read from L2 a few hundred KB later
+ * instead of two full planes (2.6 MB at 128^3) later.  Measured with FP64
+ * links at 128^3: over-relaxation launch -10 %, heat-bath -5 %.  At 256^3
+ * (128 blocks per plane) the L2-miss bytes of a heat-bath launch drop from
+ * 1.55 to 1.10 times the compulsory bytes, the t-neighbour planes no longer
+ * re-fetched through the Infinity Cache, but the launch gains only 1.7 %:
+ * there the time is set by the DRAM access pattern itself (a copy of the
+ * kernel with its arithmetic removed takes 99 % of its time).  FP32 links
+ * (half the footprint) gain nothing from it and keep the slab order.
+ * Strips are 16 blocks wide where the plane's block count allows it, else
+ * 8: at 128^3 (32 blocks per plane) 16 against 8 gave heat-bath -4 %, step
+ * -2 %; 4 and 32 were slower than 8, and at 256^3 8 was 5 % slower than
+ * 16.
+ *
+ * ORDER_COLUMNS (thin lattices, Lt <= 32, with bpp a multiple of 8): a slab
+ * of Lt/8 <= 4 planes would send half or more of an XCD's t-neighbour reads
+ * to another XCD's planes.  Instead XCD k takes the k-th eighth of every
+ * plane's blocks (a column of x-rows through all t) and walks it t-major;
+ * only the x-neighbour rows at the column's two edges belong to other
+ * XCDs.  Measured with FP64 links against ORDER_SLAB: 32x256x256
+ * heat-bath -10 %, over-relaxation -30 %; 16x256x256 -13 % and -25 %
+ * (one GPU's share of 256^3 slab-decomposed over 8 or 16 GPUs); on 256^3
+ * itself it was 2.5 % slower. */
+enum { ORDER_SLAB = 0, ORDER_STRIPS = 1, ORDER_COLUMNS = 2 };
+
+
+__device__ static inline __attribute__((always_inline)) uint32_t su2_block_of(uint32_t order, uint32_t strip,
+                                                                              uint32_t bpp)
+{
+    const uint32_t b = blockIdx.x, xcd = b & 7u, j = b >> 3, per = gridDim.x >> 3;
+    if (order == ORDER_COLUMNS) {
+        const uint32_t W = bpp >> 3; /* blocks of a plane in this XCD's column */
+        const uint32_t tl = j / W;
+        return tl * bpp + xcd * W + (j - tl * W);
+    }
+    if (order == ORDER_SLAB) return xcd * per + j;
+    const uint32_t T = per / bpp; /* t-planes of this XCD's slab */
+    const uint32_t s = j / (T * strip), r = j - s * (T * strip);
+    const uint32_t tl = r / strip, c = r - tl * strip;
+    return (xcd * T + tl) * bpp + s * strip + c;
+}
+
+/* The block order of a launch over the half-sites in blocks of `block`
+ * (see above): columns or strips for FP64 links (f64) when the lattice tiles
+ * evenly, else the slab order.  bpp = blocks per t-plane. */
+static inline void su2_pick_block_order(const su2_kgeom &g, uint32_t block, bool f64, uint32_t *order,
+                                        uint32_t *strip, uint32_t *bpp)
+{
+    const uint32_t nb = su2_grid_blocks(g.Vh, block);
+    *order = ORDER_SLAB;
+    *strip = 0;
+    *bpp = g.plane / block;
+    const uint32_t w = (*bpp % 16u == 0) ? 16u : 8u;
+    const bool whole = g.plane % block == 0 && nb * block == g.Vh;
+    if (f64 && whole && g.Lt <= 32u && *bpp % 8u == 0) {
+        *order = ORDER_COLUMNS;
+    } else if (f64 && whole && *bpp % w == 0 && *bpp <= 128 && (g.Vh / block) % (8u * *bpp) == 0) {
+        *order = ORDER_STRIPS;
+        *strip = w;
+    }
+}
+
 /* Half-site index of (t, x, y). */
 __device__ static inline __attribute__((always_inline)) uint32_t su2_hidx(const su2_kgeom &g, uint32_t t, uint32_t x,
                                                                       uint32_t y)
@@ -74,12 +148,20 @@ __device__ static inline __attribute__((always_inline)) uint32_t su2_hidx(const 
 
 /* The staple A of U_mu at half-site h of parity p; returns 0 (and leaves A
  * alone) for a link the boundary condition does not update (PCC: x and t
- * links at y = 0).  BC = SU2_K_BC_ANY takes the boundary from bc. */
-template <int MU, int BC, typename LinkT>
+ * links at y = 0).  BC = SU2_K_BC_ANY takes the boundary from bc.
+ *
+ * MEAS (MU = y, parity 0 only): also the six owned plaquettes of the
+ * thread's site m (su2_observe.h) into *meas, from the field as it is before
+ * this launch writes: they reuse the staple's twelve links and read U_y(m)
+ * and four more; the writes of the launch touch U_y at parity 0 only, and
+ * U_y(m) only in this thread, after this returns. */
+template <int MU, int BC, typename LinkT, bool MEAS = false>
 __device__ static inline __attribute__((always_inline)) int su2_site_staple(const LinkT *__restrict__ U,
                                                                             const su2_kgeom &g, int parity,
-                                                                            uint32_t h, su2_q *Aout, int bc = BC)
+                                                                            uint32_t h, su2_q *Aout, int bc = BC,
+                                                                            su2_owner_plaq *meas = nullptr)
 {
+    static_assert(!MEAS || MU == SU2_DIR_Y, "a measuring launch is a y launch");
     typedef su2_link_io<LinkT> io;
     const bool pcc = BC == SU2_K_BC_PCC || (BC == SU2_K_BC_ANY && bc == SU2_K_BC_PCC);
     const bool pmc = BC == SU2_K_BC_PMC || (BC == SU2_K_BC_ANY && bc == SU2_K_BC_PMC);
@@ -110,6 +192,11 @@ __device__ static inline __attribute__((always_inline)) int su2_site_staple(cons
     const uint32_t ymu = (MU == SU2_DIR_Y) ? yp : y;
     const uint32_t h_pmu = su2_hidx(g, tmu, xmu, ymu);
 
+    /* MEAS: U_y(m) before this launch overwrites it, and the links the x-t
+     * plaquettes share with the staple */
+    su2_q uy_m, ux_m, ux_mx, ut_m;
+    if constexpr (MEAS) uy_m = io::load(Us + SU2_DIR_Y * Vh + h);
+
     su2_q A = su2_q_zero();
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -121,6 +208,8 @@ __device__ static inline __attribute__((always_inline)) int su2_site_staple(cons
             su2_hidx(g, NU == SU2_DIR_T ? tm : tmu, NU == SU2_DIR_X ? xm : xmu, NU == SU2_DIR_Y ? ym : ymu);
 
         su2_q up, dn;
+        /* the six links in double, for a measuring launch */
+        su2_q l0, l1, l2, l3, l4, l5;
         if constexpr (sizeof(LinkT) == sizeof(su2_l32)) {
             /* FP32 links: the paths multiplied in float (su2_quat.h) */
             const su2_qf u_nu_pmu = io::loadf(Uo + NU * Vh + h_pmu);
@@ -131,6 +220,14 @@ __device__ static inline __attribute__((always_inline)) int su2_site_staple(cons
             const su2_qf u_nu_mnu = io::loadf(Uo + NU * Vh + h_mnu);
             up = su2_q_of_qf(su2_qf_mul_nd(su2_qf_mul_nd(u_nu_pmu, u_mu_pnu), u_nu_n));
             dn = su2_q_of_qf(su2_qf_mul(su2_qf_mul_dd(u_nu_pmu_mnu, u_mu_mnu), u_nu_mnu));
+            if constexpr (MEAS) {
+                l0 = su2_q_of_qf(u_nu_pmu);
+                l1 = su2_q_of_qf(u_mu_pnu);
+                l2 = su2_q_of_qf(u_nu_n);
+                l3 = su2_q_of_qf(u_nu_pmu_mnu);
+                l4 = su2_q_of_qf(u_mu_mnu);
+                l5 = su2_q_of_qf(u_nu_mnu);
+            }
         } else {
             const su2_q u_nu_pmu = io::load(Uo + NU * Vh + h_pmu);
             const su2_q u_mu_pnu = io::load(Uo + MU * Vh + h_pnu);
@@ -140,6 +237,31 @@ __device__ static inline __attribute__((always_inline)) int su2_site_staple(cons
             const su2_q u_nu_mnu = io::load(Uo + NU * Vh + h_mnu);
             up = su2_q_mul_nd(su2_q_mul_nd(u_nu_pmu, u_mu_pnu), u_nu_n);
             dn = su2_q_mul(su2_q_mul_dd(u_nu_pmu_mnu, u_mu_mnu), u_nu_mnu);
+            if constexpr (MEAS) {
+                l0 = u_nu_pmu;
+                l1 = u_mu_pnu;
+                l2 = u_nu_n;
+                l3 = u_nu_pmu_mnu;
+                l4 = u_mu_mnu;
+                l5 = u_nu_mnu;
+            }
+        }
+        if constexpr (MEAS) {
+            /* k = 0 (nu = x): l0..l5 = U_x(m+y), U_y(m+x), U_x(m), U_x(m+y-x),
+             * U_y(m-x), U_x(m-x): a(m) and a(m-x); k = 1 (nu = t) likewise
+             * b(m) and b(m-t) */
+            const double fwd = su2_plaq_term(uy_m, l0, l1, l2);
+            const double bwd = su2_plaq_term(l4, l3, uy_m, l5);
+            if (k == 0) {
+                meas->a = fwd;
+                meas->a1 = bwd;
+                ux_m = l2;
+                ux_mx = l5;
+            } else {
+                meas->b = fwd;
+                meas->b2 = bwd;
+                ut_m = l2;
+            }
         }
         if (pmc && MU != SU2_DIR_Y && NU == SU2_DIR_Y) {
             /* A is never -0 here, so adding +0 leaves it bit-identical. */
@@ -149,22 +271,153 @@ __device__ static inline __attribute__((always_inline)) int su2_site_staple(cons
         A = su2_q_add(A, up);
         A = su2_q_add(A, dn);
     }
+    if constexpr (MEAS) {
+        /* c(m) = P_xt(m) and c(m-x) with U_t(m+x), U_x(m+t), U_x(m+t-x), U_t(m-x) */
+        const uint32_t h_px = su2_hidx(g, t, xp, y), h_pt = su2_hidx(g, tp, x, y);
+        const uint32_t h_mx = su2_hidx(g, t, xm, y), h_ptmx = su2_hidx(g, tp, xm, y);
+        const su2_q ut_px = io::load(Uo + SU2_DIR_T * Vh + h_px);
+        const su2_q ux_pt = io::load(Uo + SU2_DIR_X * Vh + h_pt);
+        const su2_q ux_ptmx = io::load(Us + SU2_DIR_X * Vh + h_ptmx);
+        const su2_q ut_mx = io::load(Uo + SU2_DIR_T * Vh + h_mx);
+        meas->c = su2_plaq_term(ux_m, ut_px, ux_pt, ut_m);
+        meas->c1 = su2_plaq_term(ux_mx, ut_m, ux_ptmx, ut_mx);
+        meas->edge = y == g.Ly - 1u;
+    }
     *Aout = A;
     return 1;
 }
 
-template <int MU, int BC, int UPD, typename LinkT>
+/* The owned plaquettes of parity-0 half-site h (su2_observe.h), for an
+ * en_av launch of its own: the same links as a measuring y launch reads,
+ * loaded as doubles.  PART 0: a(m), a(m-x), b(m) (ten links); PART 1:
+ * b(m-t), c(m), c(m-x) (eleven links); PART 2: all six.  Two threads that
+ * split a site between them (parts 0 and 1) hold between them exactly the
+ * values one thread computes alone. */
+template <int PART, typename LinkT>
+__device__ static inline __attribute__((always_inline)) void su2_owner_terms(const LinkT *__restrict__ U,
+                                                                             const su2_kgeom &g, uint32_t h,
+                                                                             su2_owner_plaq *meas)
+{
+    typedef su2_link_io<LinkT> io;
+    const uint32_t q = su2_div(h, g.div_lyh);
+    const uint32_t yh = h - q * g.Lyh;
+    const uint32_t t = su2_div(q, g.div_lx);
+    const uint32_t x = q - t * g.Lx;
+    const uint32_t y = 2u * yh + ((t + x) & 1u); /* parity 0 */
+    const uint32_t yp = (y + 1u == g.Ly) ? 0u : y + 1u;
+    const uint32_t xp = (x + 1u == g.Lx) ? 0u : x + 1u;
+    const uint32_t xm = (x == 0u) ? g.Lx - 1u : x - 1u;
+    const uint32_t tp = (t + 1u == g.Lt) ? 0u : t + 1u;
+    const uint32_t tm = (t == 0u) ? g.Lt - 1u : t - 1u;
+    const size_t Vh = g.Vh;
+    const LinkT *U0 = U, *U1 = U + 3 * Vh; /* parity 0, parity 1 */
+    const uint32_t h_py = su2_hidx(g, t, x, yp), h_px = su2_hidx(g, t, xp, y), h_pt = su2_hidx(g, tp, x, y);
+    const uint32_t h_mx = su2_hidx(g, t, xm, y), h_mt = su2_hidx(g, tm, x, y);
+    const su2_q uy = io::load(U0 + SU2_DIR_Y * Vh + h);
+    const su2_q ut = io::load(U0 + SU2_DIR_T * Vh + h);
+    if constexpr (PART != 1) {
+        /* y-x: a(m), a(m-x); y-t: b(m) */
+        const uint32_t h_pymx = su2_hidx(g, t, xm, yp);
+        const su2_q ux = io::load(U0 + SU2_DIR_X * Vh + h);
+        const su2_q ux_py = io::load(U1 + SU2_DIR_X * Vh + h_py), uy_px = io::load(U1 + SU2_DIR_Y * Vh + h_px);
+        const su2_q ux_pymx = io::load(U0 + SU2_DIR_X * Vh + h_pymx), uy_mx = io::load(U1 + SU2_DIR_Y * Vh + h_mx);
+        const su2_q ux_mx = io::load(U1 + SU2_DIR_X * Vh + h_mx);
+        const su2_q ut_py = io::load(U1 + SU2_DIR_T * Vh + h_py), uy_pt = io::load(U1 + SU2_DIR_Y * Vh + h_pt);
+        meas->a = su2_plaq_term(uy, ux_py, uy_px, ux);
+        meas->a1 = su2_plaq_term(uy_mx, ux_pymx, uy, ux_mx);
+        meas->b = su2_plaq_term(uy, ut_py, uy_pt, ut);
+    }
+    if constexpr (PART != 0) {
+        /* y-t: b(m-t); x-t: c(m), c(m-x) */
+        const uint32_t h_pymt = su2_hidx(g, tm, x, yp), h_ptmx = su2_hidx(g, tp, xm, y);
+        const su2_q ux = io::load(U0 + SU2_DIR_X * Vh + h);
+        const su2_q ut_pymt = io::load(U0 + SU2_DIR_T * Vh + h_pymt), uy_mt = io::load(U1 + SU2_DIR_Y * Vh + h_mt);
+        const su2_q ut_mt = io::load(U1 + SU2_DIR_T * Vh + h_mt);
+        meas->b2 = su2_plaq_term(uy_mt, ut_pymt, uy, ut_mt);
+        const su2_q ut_px = io::load(U1 + SU2_DIR_T * Vh + h_px), ux_pt = io::load(U1 + SU2_DIR_X * Vh + h_pt);
+        const su2_q ux_mx = io::load(U1 + SU2_DIR_X * Vh + h_mx);
+        const su2_q ux_ptmx = io::load(U0 + SU2_DIR_X * Vh + h_ptmx), ut_mx = io::load(U1 + SU2_DIR_T * Vh + h_mx);
+        meas->c = su2_plaq_term(ux, ut_px, ux_pt, ut);
+        meas->c1 = su2_plaq_term(ux_mx, ut, ux_ptmx, ut_mx);
+    }
+    meas->edge = y == g.Ly - 1u;
+}
+
+/* Workgroup size of the en_av partial launches (single-lattice and
+ * batched): two threads per owned site with FP64 links, one with FP32
+ * links (su2_en_av_owner_block). */
+static inline uint32_t su2_en_av_threads(int f32) { return f32 ? SU2_EN_BLOCK : 2 * SU2_EN_BLOCK; }
+
+/* One logical block of en_av pass 1 (owners of parity-0 half-sites
+ * [b * SU2_EN_BLOCK, ...)): partials 2b, 2b + 1 into work, the same bits as a
+ * measuring sweep launch writes.  FP64 links: each owned site takes two
+ * threads of a 2 * SU2_EN_BLOCK workgroup, the first half computing a(m),
+ * a(m-x), b(m), the second b(m-t), c(m), c(m-x) (ten and eleven links
+ * instead of seventeen in one thread); the second half hands its three
+ * values over in LDS and the first half forms and reduces the owners' sums
+ * exactly as one thread per owner does.  128^3 FP64: 40 us against 50 us
+ * with one thread per owner; FP32 links: 26 us against 24 us, so they keep
+ * one thread per owner.  Every thread of the workgroup calls this. */
+template <typename LinkT>
+__device__ static inline __attribute__((always_inline)) void su2_en_av_owner_block(const LinkT *__restrict__ U,
+                                                                                   const su2_kgeom &g, int pmc,
+                                                                                   uint32_t b,
+                                                                                   double *__restrict__ work)
+{
+    constexpr bool split = sizeof(LinkT) == sizeof(su2_l64);
+    const uint32_t o = threadIdx.x % SU2_EN_BLOCK, second = threadIdx.x / SU2_EN_BLOCK;
+    const uint32_t h = b * SU2_EN_BLOCK + o;
+    su2_owner_plaq m = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+    double e = 0.0, pl = 0.0;
+    if constexpr (split) {
+        __shared__ double xch[3][SU2_EN_BLOCK];
+        if (h < g.Vh) {
+            if (second)
+                su2_owner_terms<1, LinkT>(U, g, h, &m);
+            else
+                su2_owner_terms<0, LinkT>(U, g, h, &m);
+        }
+        if (second) {
+            xch[0][o] = m.b2;
+            xch[1][o] = m.c;
+            xch[2][o] = m.c1;
+        }
+        __syncthreads();
+        if (!second && h < g.Vh) {
+            m.b2 = xch[0][o];
+            m.c = xch[1][o];
+            m.c1 = xch[2][o];
+            su2_owner_finish(m, pmc, &e, &pl);
+        }
+    } else {
+        if (h < g.Vh) {
+            su2_owner_terms<2, LinkT>(U, g, h, &m);
+            su2_owner_finish(m, pmc, &e, &pl);
+        }
+    }
+    /* the owners' sums (a second half adds zeros to wavefront totals that
+     * are not read) */
+    su2_block_sum2(e, pl, SU2_EN_BLOCK);
+    if (threadIdx.x == 0) {
+        work[2 * b] = e;
+        work[2 * b + 1] = pl;
+    }
+}
+
+/* MEAS: the launch also measures the field it starts from (su2_site_staple). */
+template <int MU, int BC, int UPD, typename LinkT, bool MEAS = false>
 __device__ static inline __attribute__((always_inline)) void su2_sweep_site(LinkT *__restrict__ U,
                                                                             uint32_t *__restrict__ rng,
                                                                             int *__restrict__ accept,
                                                                             const su2_kgeom &g, const su2_kphys &ph,
                                                                             int parity, uint32_t h, int bc = BC,
-                                                                            int acc_mode = SU2_K_ACC_LAUNCH)
+                                                                            int acc_mode = SU2_K_ACC_LAUNCH,
+                                                                            su2_owner_plaq *meas = nullptr)
 {
     typedef su2_link_io<LinkT> io;
 
     su2_q A;
-    if (!su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A, bc)) {
+    if (!su2_site_staple<MU, BC, LinkT, MEAS>(U, g, parity, h, &A, bc, meas)) {
         /* a link the boundary does not update (PCC x/t at y = 0): its
          * half-site's sweep count is the y link's update alone */
         if (acc_mode == SU2_K_ACC_SWEEP) atomicAdd(&accept[(size_t)parity * g.Vh + h], 1);
@@ -224,11 +477,11 @@ struct su2_kp_queue {
     float lf[256];
 };
 
-template <int MU, int BC, typename LinkT, bool EARLY_RNG = true, bool STREAM_RNG = false>
+template <int MU, int BC, typename LinkT, bool EARLY_RNG = true, bool STREAM_RNG = false, bool MEAS = false>
 __device__ static inline __attribute__((always_inline)) void
 su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept,
                      const su2_kgeom &g, const su2_kphys &ph, int parity, uint32_t h, int in_range,
-                     su2_kp_queue &q, int bc = BC, int acc_mode = SU2_K_ACC_LAUNCH)
+                     su2_kp_queue &q, int bc = BC, int acc_mode = SU2_K_ACC_LAUNCH, su2_owner_plaq *meas = nullptr)
 {
     typedef su2_link_io<LinkT> io;
 
@@ -264,7 +517,7 @@ su2_sweep_site_hb_wg(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__r
         }
     }
     su2_q A = su2_q_zero();
-    const int active = in_range ? su2_site_staple<MU, BC, LinkT>(U, g, parity, h, &A, bc) : 0;
+    const int active = in_range ? su2_site_staple<MU, BC, LinkT, MEAS>(U, g, parity, h, &A, bc, meas) : 0;
     const size_t Vh = g.Vh;
     float inv = 0.0f, lf = 0.0f;
     int done = 1;

@@ -146,6 +146,14 @@ class Lattice:
     def mc_up(self, bc: int, algo: int = L.ALGO_HEATBATH, update: int = L.UPDATE_LINK) -> None:
         (self.mc_up_per, self.mc_up_pcc, self.mc_up_pmc)[bc](algo, update)
 
+    def mc_up_measured(self, bc: int, res_ptr: int, res2_ptr: int, algo: int = L.ALGO_HEATBATH,
+                       update: int = L.UPDATE_LINK) -> None:
+        """en_av of the current field into caller device slots, then one sweep
+        (su2_mc_up_measured): the results of en_av_into followed by mc_up."""
+        L.check(self.lib.su2_mc_up_measured(self.lp, self.phi.ptr, self.precision, self.accept_buf.ptr,
+                                            self.states.ptr, bc, algo, update, C.c_void_p(res_ptr),
+                                            C.c_void_p(res2_ptr), self.work.ptr, self.stream), "mc_up_measured")
+
     def mc_update(self, bc: int, algo: int, update: int, mu: int, parity: int) -> None:
         L.check(self.lib.su2_mc_update(self.lp, self.phi.ptr, self.precision, self.accept_buf.ptr, self.states.ptr,
                                        bc, algo, update, mu, parity, self.stream), "mc_update")

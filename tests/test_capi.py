@@ -55,7 +55,7 @@ def test_lattice_init_and_sizes():
     assert lib.su2_field_bytes(C.byref(l), 1) == 12 * 2097152 * 4
     assert lib.su2_rng_bytes(C.byref(l)) == 24 * 1048576
     assert lib.su2_accept_bytes(C.byref(l)) == 4 * 2097152
-    assert lib.su2_en_av_work_bytes(C.byref(l)) == 16 * 8192
+    assert lib.su2_en_av_work_bytes(C.byref(l)) == 16 * 4096  # two doubles per block of 256 half-sites
 
 
 @pytest.mark.parametrize("ext", [(15, 16, 16), (16, 15, 16), (16, 16, 3), (0, 4, 4), (2, 2, -2)])

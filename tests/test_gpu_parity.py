@@ -243,3 +243,69 @@ def test_randomised_cases(gpu, oracle_mod, case):
                  updates=c["updates"], normalize_at=(2,))
     finally:
         lib.su2_set_sweep_block(0)
+
+
+def _res_slots(q, n):
+    buf = q.DeviceBuffer(16 * n)
+    buf.zero()
+    return buf
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("shape,bc,algo,update", [
+    ((16, 12, 20), 0, 1, 0),   # heat-bath, padded last block (V/2 = 1920)
+    ((16, 12, 20), 1, 0, 0),   # PCC Metropolis
+    ((16, 12, 20), 2, 1, 0),   # PMC heat-bath
+    ((16, 12, 20), 2, 0, 1),   # PMC over-relaxation
+    ((64, 64, 64), 0, 1, 0),   # FP64: strip block order
+    ((8, 64, 64), 2, 1, 0),    # FP64: x-column block order
+])
+def test_measured_sweep_equals_en_av_then_sweep(gpu, oracle_mod, prec, shape, bc, algo, update):
+    """su2_mc_up_measured (the measurement carried by the sweep's first
+    launch) against su2_en_av followed by the same sweep: links, RNG states,
+    acceptance counters and both measured values bit for bit, three rounds in
+    a row; the measurement also against the checker's en_av to 1e-12, and
+    with a sweep block of 128 threads (the measurement then runs on its own)
+    the same bits again."""
+    q = gpu
+    Lt, Lx, Ly = shape
+    pmc = bc == 2
+    _say(f"[measured] {shape} bc={bc} algo={algo} update={update} prec={prec}")
+    gs = [q.Lattice(Lt, Lx, Ly, 3.0, 0.4, 2, precision=prec) for _ in range(3)]
+    o = oracle_mod.Oracle(Lt, Lx, Ly, 3.0, 0.4, 2, f32=(prec == 1))
+    slots = [_res_slots(q, 3) for _ in gs]
+    lib = q.load_library()
+    try:
+        for g in gs:
+            g.setup_kernel(77)
+            g.start(0, bc)
+        for r in range(3):
+            ga, gb, gc = gs
+            ga.en_av_into(pmc, slots[0].value + 16 * r, slots[0].value + 16 * r + 8)
+            ga.mc_up(bc, algo, update)
+            gb.mc_up_measured(bc, slots[1].value + 16 * r, slots[1].value + 16 * r + 8, algo, update)
+            lib.su2_set_sweep_block(128)
+            try:
+                gc.mc_up_measured(bc, slots[2].value + 16 * r, slots[2].value + 16 * r + 8, algo, update)
+            finally:
+                lib.su2_set_sweep_block(0)
+            pa = ga.field_ref()
+            for g, name in ((gb, "measured"), (gc, "measured, block 128")):
+                assert _bits_equal(g.field_ref(), pa), f"round {r} {name}: links differ"
+                assert np.array_equal(g.rng_state(), ga.rng_state()), f"round {r} {name}: RNG differs"
+                assert np.array_equal(g.accept(), ga.accept()), f"round {r} {name}: acceptance differs"
+        ra, rb, rc = (s.download(np.float64, 6).reshape(3, 2) for s in slots)
+        assert _bits_equal(rb, ra) and _bits_equal(rc, ra), (ra, rb, rc)
+        # the measurement of round 0 is of the start field
+        g0 = q.Lattice(Lt, Lx, Ly, 3.0, 0.4, 2, precision=prec)
+        try:
+            g0.setup_kernel(77)
+            g0.start(0, bc)
+            o.phi[:] = g0.field_ref()
+        finally:
+            g0.close()
+        eo, po = o.en_av(pmc)
+        assert abs(ra[0, 0] - eo) <= REL_TOL * abs(eo) and abs(ra[0, 1] - po) <= REL_TOL * abs(po), (ra[0], eo, po)
+    finally:
+        for g in gs:
+            g.close()
