@@ -89,6 +89,28 @@ __device__ static inline bool measure_now(const su2_bdesc &d, uint32_t s)
     return (k % (uint32_t)d.n_cor) == (uint32_t)d.n_cor - 1u;
 }
 
+/* The measurement of sweep s is carried by the first update launch of the
+ * lattice's next tick (carried_in), which forms the en_av partials of the
+ * field it starts from: no launch changes the field in between, since a
+ * reunitarisation before the sweep happens only during thermalisation.  It
+ * needs workgroups of SU2_EN_BLOCK threads in the sweep launches
+ * (carry_on).  The last measurement of a lattice, and every one without
+ * carry_on, runs as en_av at the end of its tick (measure_at_end). */
+__device__ static inline uint32_t total_sweeps(const su2_bdesc &d)
+{
+    return (uint32_t)d.n_therm + (uint32_t)d.n_meas * (uint32_t)d.n_cor;
+}
+
+__device__ static inline bool carried_in(const su2_bdesc &d, uint32_t s, int carry_on)
+{
+    return carry_on && s >= 1u && measure_now(d, s - 1u);
+}
+
+__device__ static inline bool measure_at_end(const su2_bdesc &d, uint32_t s, int carry_on)
+{
+    return measure_now(d, s) && (!carry_on || s + 1u == total_sweeps(d));
+}
+
 __device__ static inline bool normalize_pre(const su2_bdesc &d, uint32_t s)
 {
     return s < (uint32_t)d.n_therm && ((s + 1u) % (uint32_t)d.nevery) == 0u;
@@ -116,7 +138,8 @@ __device__ static inline constexpr int accept_once_per_sweep()
 }
 
 template <int MU, int ALGO, typename LinkT>
-__device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t h, int in_range, su2_kp_queue &q)
+__device__ static inline __attribute__((always_inline)) void site_link(const su2_bdesc &d, int parity, uint32_t h,
+                                                                       int in_range, su2_kp_queue &q)
 {
     if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
         /* early RNG load for FP32 links only (measured: batched FP32 -4.5 %,
@@ -128,13 +151,39 @@ __device__ static inline void site_link(const su2_bdesc &d, int parity, uint32_t
                                                             d.bc);
 }
 
+/* site_link that also measures the field it starts from (MU = y, parity 0,
+ * workgroups of SU2_EN_BLOCK threads): logical en_av block lb of the lattice
+ * goes to its carried partials, the bits su2_en_av_owner_block writes. */
+template <int MU, int ALGO, typename LinkT>
+__device__ static inline __attribute__((always_inline)) void site_link_measured(const su2_bdesc &d, int parity,
+                                                                                uint32_t h, int in_range,
+                                                                                su2_kp_queue &q, uint32_t lb)
+{
+    su2_owner_plaq m = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+    if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
+        su2_sweep_site_hb_wg<MU, SU2_K_BC_ANY, LinkT, sizeof(LinkT) == sizeof(su2_l32), false, true>(
+            (LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, in_range, q, d.bc, accept_once_per_sweep<MU>(), &m);
+    else if (in_range)
+        su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_METRO, LinkT, true>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity,
+                                                                  h, d.bc, SU2_K_ACC_LAUNCH, &m);
+    double e = 0.0, pl = 0.0;
+    if (in_range) su2_owner_finish(m, d.bc == SU2_K_BC_PMC, &e, &pl);
+    su2_block_sum2(e, pl, SU2_EN_BLOCK);
+    if (threadIdx.x == 0) {
+        d.partc[2 * lb] = e;
+        d.partc[2 * lb + 1] = pl;
+    }
+    __syncthreads(); /* su2_block_sum2's shared array is reused by the next block */
+}
+
 /* OR = 0: link update; ALGO is the algorithm class of the resident lattices
  * (all heat-bath, all Metropolis, or mixed), so the common homogeneous case
  * carries one update body.  OR = 1: over-relaxation pass number or_pass
  * (lattices with fewer over-relaxation sweeps per tick skip it). */
-template <int MU, int OR, int ALGO, typename LinkT>
+template <int MU, int OR, int ALGO, typename LinkT, bool MEAS = false>
 __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
-                                                const uint32_t *__restrict__ B, int parity, int or_pass)
+                                                const uint32_t *__restrict__ B, int parity, int or_pass,
+                                                const uint32_t *__restrict__ ctr)
 {
     __shared__ uint32_t s_begin[SU2_BATCH_LDS_DESC];
     const uint32_t n = H->n_active;
@@ -159,8 +208,15 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
         for (uint32_t vb = vb0; vb < vend; vb += step) {
             /* everything up to the update is uniform in the workgroup */
             const su2_bdesc &d = D[find_desc(B, s_begin, lds, n, vb)];
-            const uint32_t h = (vb - d.sw_begin) * blockDim.x + threadIdx.x;
+            const uint32_t lb = vb - d.sw_begin;
+            const uint32_t h = lb * blockDim.x + threadIdx.x;
             const int in_range = h < d.g.Vh;
+            if constexpr (MEAS) {
+                if (carried_in(d, ctr[d.slot], 1)) {
+                    site_link_measured<MU, ALGO, LinkT>(d, parity, h, in_range, q, lb);
+                    continue;
+                }
+            }
             site_link<MU, ALGO, LinkT>(d, parity, h, in_range, q);
         }
     }
@@ -197,10 +253,12 @@ __global__ __launch_bounds__(256) void k_bnormalize(const su2_bhead *__restrict_
 /* en_av pass 1 for the lattices that measure this tick; logical block b of
  * a lattice has the single-lattice layout and bits (su2_en_av_owner_block). */
 template <typename LinkT>
-__global__ __launch_bounds__(2 * SU2_EN_BLOCK) void k_ben_av_partial(const su2_bhead *__restrict__ H,
-                                                        const su2_bdesc *__restrict__ D,
-                                                        const uint32_t *__restrict__ B, uint32_t max_desc,
-                                                        const uint32_t *__restrict__ ctr)
+__device__ static inline __attribute__((always_inline)) void ben_av_partial(const su2_bhead *__restrict__ H,
+                                                                            const su2_bdesc *__restrict__ D,
+                                                                            const uint32_t *__restrict__ B,
+                                                                            uint32_t max_desc,
+                                                                            const uint32_t *__restrict__ ctr,
+                                                                            int carry_on)
 {
     __shared__ uint32_t s_begin[SU2_BATCH_LDS_DESC];
     const uint32_t n = H->n_active;
@@ -211,40 +269,63 @@ __global__ __launch_bounds__(2 * SU2_EN_BLOCK) void k_ben_av_partial(const su2_b
     const bool lds = stage_begins(Bk, n, s_begin);
     for (uint32_t vb = vb0; vb < vend; vb += step) {
         const su2_bdesc &d = D[find_desc(Bk, s_begin, lds, n, vb)];
-        if (!measure_now(d, ctr[d.slot])) continue;
+        if (!measure_at_end(d, ctr[d.slot], carry_on)) continue;
         su2_en_av_owner_block<LinkT>((const LinkT *)d.U, d.g, d.bc == SU2_K_BC_PMC, vb - d.en_begin, d.part);
         __syncthreads(); /* su2_block_sum2's shared array is reused by the next block */
     }
 }
 
-/* en_av pass 2: workgroup j (1024 threads) adds lattice j's partials in the
- * single-lattice order and accumulates sum / V into its measurement slot.
- * It is the last launch of a measuring tick, so it also ends the tick:
- * every lattice's sweep counter is read once and then advanced by one. */
+/* FP64 links, two threads per owner: held to 80 VGPRs (6 waves per SIMD, 3
+ * workgroups per CU instead of 2 at 85; no spill), 2 % off the 64-lattice
+ * ensemble.  FP32 links, one thread per owner, would spill under that cap. */
+__global__ __launch_bounds__(2 * SU2_EN_BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_ben_av_partial64(
+    const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D, const uint32_t *__restrict__ B,
+    uint32_t max_desc, const uint32_t *__restrict__ ctr, int carry_on)
+{
+    ben_av_partial<su2_l64>(H, D, B, max_desc, ctr, carry_on);
+}
+
+__global__ __launch_bounds__(SU2_EN_BLOCK) void k_ben_av_partial32(const su2_bhead *__restrict__ H,
+                                                                   const su2_bdesc *__restrict__ D,
+                                                                   const uint32_t *__restrict__ B, uint32_t max_desc,
+                                                                   const uint32_t *__restrict__ ctr, int carry_on)
+{
+    ben_av_partial<su2_l32>(H, D, B, max_desc, ctr, carry_on);
+}
+
+/* Adds one measurement of lattice d, pass 2 over its partials part, into
+ * measurement slot m: the single-lattice order and bits. */
+__device__ static inline void add_measurement(const su2_bdesc &d, const double *part, uint32_t m)
+{
+    double te, tp;
+    su2_en_final_partials(part, d.en_nb, &te, &tp);
+    su2_block_sum2(te, tp, SU2_EN_FINAL_BLOCK);
+    if (threadIdx.x == 0) {
+        const double invV = 1.0 / (2.0 * (double)d.g.Vh);
+        d.resE[m] += te * invV;
+        d.resP[m] += tp * invV;
+    }
+    __syncthreads(); /* su2_block_sum2's shared array is reused by the next call */
+}
+
+/* en_av pass 2: workgroup j (1024 threads) adds lattice j's carried
+ * measurement (of the sweep before this tick, partials from the tick's
+ * first update launch) and its end-of-tick one, each in the single-lattice
+ * order, into their measurement slots.  It is the last launch of a
+ * measuring tick, so it also ends the tick: every lattice's sweep counter
+ * is read once and then advanced by one. */
 __global__ __launch_bounds__(1024) void k_ben_av_final(const su2_bhead *__restrict__ H,
                                                        const su2_bdesc *__restrict__ D,
-                                                       uint32_t *__restrict__ ctr)
+                                                       uint32_t *__restrict__ ctr, int carry_on)
 {
     const uint32_t n = H->n_active;
     for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
         const su2_bdesc &d = D[j];
         const uint32_t s = ctr[d.slot];
         __syncthreads(); /* every wave has read the counter before lane 0 advances it */
-        if (!measure_now(d, s)) {
-            if (threadIdx.x == 0) ctr[d.slot] = s + 1u;
-            continue;
-        }
-        const uint32_t m = (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor;
-        double te, tp;
-        su2_en_final_partials(d.part, d.en_nb, &te, &tp); /* the single-lattice order */
-        su2_block_sum2(te, tp, SU2_EN_FINAL_BLOCK);
-        if (threadIdx.x == 0) {
-            const double invV = 1.0 / (2.0 * (double)d.g.Vh);
-            d.resE[m] += te * invV;
-            d.resP[m] += tp * invV;
-            ctr[d.slot] = s + 1u;
-        }
-        __syncthreads();
+        if (carried_in(d, s, carry_on)) add_measurement(d, d.partc, (s - 1u - (uint32_t)d.n_therm) / (uint32_t)d.n_cor);
+        if (measure_at_end(d, s, carry_on)) add_measurement(d, d.part, (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor);
+        if (threadIdx.x == 0) ctr[d.slot] = s + 1u;
     }
 }
 
@@ -259,32 +340,45 @@ __global__ __launch_bounds__(256) void k_btick_end(const su2_bhead *__restrict__
 
 template <int MU, int OR, int ALGO, typename LinkT>
 hipError_t launch_bsweep(const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, int parity, int or_pass,
-                         uint32_t grid, uint32_t block, hipStream_t st)
+                         uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *ctr, int measure)
 {
-    hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT>), dim3(grid), dim3(block), 0, st, H, D, B, parity, or_pass);
+    if constexpr (MU == SU2_DIR_Y && !OR) {
+        if (measure) {
+            hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT, true>), dim3(grid), dim3(block), 0, st, H, D, B, parity,
+                               or_pass, ctr);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT>), dim3(grid), dim3(block), 0, st, H, D, B, parity, or_pass,
+                       ctr);
     return hipGetLastError();
 }
 
 template <int OR, int ALGO, typename LinkT>
 hipError_t launch_bsweep_mu(int mu, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, int parity,
-                            int or_pass, uint32_t grid, uint32_t block, hipStream_t st)
+                            int or_pass, uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *ctr,
+                            int measure)
 {
     switch (mu) {
-    case 0: return launch_bsweep<0, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st);
-    case 1: return launch_bsweep<1, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st);
-    default: return launch_bsweep<2, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st);
+    case 0: return launch_bsweep<0, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, ctr, measure);
+    case 1: return launch_bsweep<1, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, ctr, 0);
+    default: return launch_bsweep<2, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, ctr, 0);
     }
 }
 
 template <typename LinkT>
 hipError_t launch_bsweep_t(int mu, int over_relax, int algo_class, const su2_bhead *H, const su2_bdesc *D,
-                           const uint32_t *B, int parity, int or_pass, uint32_t grid, uint32_t block, hipStream_t st)
+                           const uint32_t *B, int parity, int or_pass, uint32_t grid, uint32_t block, hipStream_t st,
+                           const uint32_t *ctr, int measure)
 {
-    if (over_relax) return launch_bsweep_mu<1, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, or_pass, grid, block, st);
+    if (over_relax)
+        return launch_bsweep_mu<1, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, or_pass, grid, block, st, ctr, 0);
     switch (algo_class) {
-    case SU2_BALGO_HB: return launch_bsweep_mu<0, SU2_BALGO_HB, LinkT>(mu, H, D, B, parity, 0, grid, block, st);
-    case SU2_BALGO_METRO: return launch_bsweep_mu<0, SU2_BALGO_METRO, LinkT>(mu, H, D, B, parity, 0, grid, block, st);
-    default: return launch_bsweep_mu<0, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, 0, grid, block, st);
+    case SU2_BALGO_HB:
+        return launch_bsweep_mu<0, SU2_BALGO_HB, LinkT>(mu, H, D, B, parity, 0, grid, block, st, ctr, measure);
+    case SU2_BALGO_METRO:
+        return launch_bsweep_mu<0, SU2_BALGO_METRO, LinkT>(mu, H, D, B, parity, 0, grid, block, st, ctr, measure);
+    default: return launch_bsweep_mu<0, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, 0, grid, block, st, ctr, measure);
     }
 }
 
@@ -292,11 +386,16 @@ hipError_t launch_bsweep_t(int mu, int over_relax, int algo_class, const su2_bhe
 
 hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, int algo_class, const su2_bhead *H,
                              const su2_bdesc *D, const uint32_t *B, uint32_t max_desc, int parity, int or_pass,
-                             uint32_t grid, uint32_t block, hipStream_t st)
+                             uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *ctr, int measure)
 {
     (void)max_desc; /* the sweep range starts are the first array of B */
-    if (f32) return launch_bsweep_t<su2_l32>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st);
-    return launch_bsweep_t<su2_l64>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st);
+    if (measure && (mu != SU2_DIR_Y || parity != 0 || over_relax || block != SU2_EN_BLOCK || !ctr))
+        return hipErrorInvalidValue;
+    if (f32)
+        return launch_bsweep_t<su2_l32>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st, ctr,
+                                        measure);
+    return launch_bsweep_t<su2_l64>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st, ctr,
+                                    measure);
 }
 
 hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B,
@@ -310,15 +409,20 @@ hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D
 }
 
 hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, uint32_t max_desc,
-                             uint32_t *ctr, uint32_t grid, uint32_t final_grid, hipStream_t st)
+                             uint32_t *ctr, uint32_t grid, uint32_t final_grid, int at_end, int carry_on,
+                             hipStream_t st)
 {
-    if (f32)
-        hipLaunchKernelGGL(k_ben_av_partial<su2_l32>, dim3(grid), dim3(su2_en_av_threads(1)), 0, st, H, D, B, max_desc, ctr);
-    else
-        hipLaunchKernelGGL(k_ben_av_partial<su2_l64>, dim3(grid), dim3(su2_en_av_threads(0)), 0, st, H, D, B, max_desc, ctr);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ben_av_final, dim3(final_grid), dim3(SU2_EN_FINAL_BLOCK), 0, st, H, D, ctr);
+    if (at_end) {
+        if (f32)
+            hipLaunchKernelGGL(k_ben_av_partial32, dim3(grid), dim3(su2_en_av_threads(1)), 0, st, H, D, B, max_desc,
+                               ctr, carry_on);
+        else
+            hipLaunchKernelGGL(k_ben_av_partial64, dim3(grid), dim3(su2_en_av_threads(0)), 0, st, H, D, B, max_desc,
+                               ctr, carry_on);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_ben_av_final, dim3(final_grid), dim3(SU2_EN_FINAL_BLOCK), 0, st, H, D, ctr, carry_on);
     return hipGetLastError();
 }
 

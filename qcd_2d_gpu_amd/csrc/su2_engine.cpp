@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "su2_batch.h"
+#include "su2_observe.h"
 #include "su2mc.h"
 
 namespace {
@@ -52,7 +53,7 @@ namespace {
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 struct job_layout {
-    size_t U, rng, acc, part, resE, resP, total;
+    size_t U, rng, acc, part, partc, resE, resP, total;
     uint32_t en_nb;
 };
 
@@ -70,6 +71,8 @@ job_layout layout_of(const su2_job &j, int precision)
     L.acc = o;
     o += align256(V * 4);
     L.part = o;
+    o += align256((size_t)L.en_nb * 16);
+    L.partc = o;
     o += align256((size_t)L.en_nb * 16);
     L.resE = o;
     o += align256((size_t)j.n_meas * 8);
@@ -135,6 +138,13 @@ bool h_measure_now(const su2_job &j, int s)
 
 bool h_normalize_pre(const su2_job &j, int s) { return s < j.n_therm && (s + 1) % nevery_of(j) == 0; }
 
+bool h_carried_in(const su2_job &j, int s, bool carry_on) { return carry_on && s >= 1 && h_measure_now(j, s - 1); }
+
+bool h_measure_at_end(const su2_job &j, int s, int total, bool carry_on)
+{
+    return h_measure_now(j, s) && (!carry_on || s + 1 == total);
+}
+
 bool h_normalize_post(const su2_job &j, int s)
 {
     if (s + 1 == j.n_therm) return true;
@@ -144,7 +154,9 @@ bool h_normalize_post(const su2_job &j, int s)
 
 /* which launches a tick needs, and their grids */
 struct tick_plan {
-    bool pre = false, post = false, meas = false;
+    /* meas: en_av at the end of the tick; carry: the first update launch
+     * measures for the lattices that carry a measurement into the tick */
+    bool pre = false, post = false, meas = false, carry = false;
     int n_or = 0;
     int algo = SU2_BALGO_MIXED;
     uint32_t g_sw = 0, g_nm = 0, g_en = 0, g_fin = 0; /* workgroups per launch kind */
@@ -155,7 +167,7 @@ struct tick_plan {
             while ((1u << l) < v) ++l;
             return l;
         };
-        const uint64_t flags = (((uint64_t)(n_or * 3 + algo) * 2 + pre) * 2 + post) * 2 + meas;
+        const uint64_t flags = ((((uint64_t)(n_or * 3 + algo) * 2 + pre) * 2 + post) * 2 + meas) * 2 + carry;
         return (flags << 24) | (lg(g_sw) << 18) | (lg(g_nm) << 12) | (lg(g_en) << 6) | lg(g_fin);
     }
 };
@@ -245,6 +257,7 @@ struct su2_engine {
             d.rng = (uint32_t *)(arena + s.off + s.lay.rng);
             d.accept = (int *)(arena + s.off + s.lay.acc);
             d.part = (double *)(arena + s.off + s.lay.part);
+            d.partc = (double *)(arena + s.off + s.lay.partc);
             d.resE = (double *)(arena + s.off + s.lay.resE);
             d.resP = (double *)(arena + s.off + s.lay.resP);
             d.g = s.g;
@@ -292,11 +305,13 @@ struct su2_engine {
     {
         tick_plan p;
         bool hb = false, metro = false;
+        const bool carry_on = block == (int)SU2_EN_BLOCK;
         for (const auto &s : slots) {
             if (!s.used || s.done) continue;
             p.pre |= h_normalize_pre(s.job, s.swept);
             p.post |= h_normalize_post(s.job, s.swept);
-            p.meas |= h_measure_now(s.job, s.swept);
+            p.meas |= h_measure_at_end(s.job, s.swept, s.total, carry_on);
+            p.carry |= h_carried_in(s.job, s.swept, carry_on);
             p.n_or = std::max(p.n_or, s.job.or_sweeps);
             (s.job.algo == SU2_ALGO_HEATBATH ? hb : metro) = true;
         }
@@ -316,14 +331,16 @@ struct su2_engine {
         if (p.pre) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 0, p.g_nm, st);
         for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
             for (int par = 0; par < 2 && e == hipSuccess; ++par)
-                e = su2_launch_bsweep(f32, mu, 0, p.algo, dH, dD, dB, mj, par, 0, p.g_sw, block, st);
+                e = su2_launch_bsweep(f32, mu, 0, p.algo, dH, dD, dB, mj, par, 0, p.g_sw, block, st, dctr,
+                                      p.carry && mu == 0 && par == 0);
         for (int pass = 0; pass < p.n_or && e == hipSuccess; ++pass)
             for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
                 for (int par = 0; par < 2 && e == hipSuccess; ++par)
                     e = su2_launch_bsweep(f32, mu, 1, p.algo, dH, dD, dB, mj, par, pass, p.g_sw, block, st);
         if (p.post && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 1, p.g_nm, st);
-        if (p.meas && e == hipSuccess) e = su2_launch_ben_av(f32, dH, dD, dB, mj, dctr, p.g_en, p.g_fin, st);
-        if (!p.meas && e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
+        if ((p.meas || p.carry) && e == hipSuccess)
+            e = su2_launch_ben_av(f32, dH, dD, dB, mj, dctr, p.g_en, p.g_fin, p.meas, block == (int)SU2_EN_BLOCK, st);
+        if (!p.meas && !p.carry && e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
         if (e != hipSuccess) return su2_set_error(SU2_EHIP, "tick launch: %s", hipGetErrorString(e));
         return SU2_OK;
     }
@@ -520,7 +537,9 @@ int su2_engine_add(su2_engine *e, const su2_job *job, int *slot)
                                                 : (job->bc == SU2_BC_PCC ? 2 : 0);
     if (he == hipSuccess) he = su2_launch_init(start, f32, base + lay.U, (uint32_t *)(base + lay.rng), s.g, e->st);
     if (he == hipSuccess) he = su2_launch_fill_i32((int *)(base + lay.acc), 0, 2 * (size_t)s.g.Vh, e->st);
-    if (he == hipSuccess) he = hipMemsetAsync(base + lay.resE, 0, lay.total - lay.resE, e->st);
+    /* the carried partials past the last sweep block stay zero (the sweep
+     * launches write only blocks that hold half-sites) */
+    if (he == hipSuccess) he = hipMemsetAsync(base + lay.partc, 0, lay.total - lay.partc, e->st);
     if (he == hipSuccess) he = hipMemsetAsync(e->dctr + id, 0, sizeof(uint32_t), e->st);
     if (he != hipSuccess) {
         e->release(off, lay.total);

@@ -112,6 +112,24 @@ def test_batch_equals_alone(gpu, oracle_mod, prec, use_graph):
         eng.close()
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_without_carried_measurement(gpu, prec):
+    """Sweep launches of 128-thread workgroups: no measurement rides on the
+    next tick's first update launch (that needs 256), each runs as en_av at
+    the end of its tick, and every job still gives the bits it gives alone."""
+    q = gpu
+    _say(f"[engine] block 128, measurements at the end of their ticks, prec={prec}")
+    jobs = mixed_jobs(q, 6, seed0=2500 + 10 * prec)
+    eng = q.Engine(precision=prec, max_jobs=8, budget_bytes=64 << 20, use_graph=True, block=128)
+    try:
+        slots = [eng.add(j) for j in jobs]
+        eng.run()
+        for s, j in zip(slots, jobs):
+            check_same(q, eng, s, j, prec, f"block 128 slot {s} {j}")
+    finally:
+        eng.close()
+
+
 def test_jobs_join_and_leave(gpu):
     """Continuous batching on the engine: jobs added while others run, one
     removed unfinished, results of the rest unchanged."""
