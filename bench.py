@@ -174,7 +174,7 @@ def gpu_bench(args):
     # sweeps between them run unbracketed, as in the step
     ns = args.roofline_sweeps
     evs = []
-    for _ in range(2 * ns + 2):
+    for _ in range(2 * max(ns, 2) + 2):
         e = C.c_void_p()
         L.check(lib.su2_event_create(C.byref(e)), "event")
         evs.append(e)
@@ -217,6 +217,19 @@ def gpu_bench(args):
             g.en_av_into(False, res.value, res.value + 8)
 
     en_ms = [t / 6 for t in bracketed(en_av6, ns)]
+
+    # the measurement as the step makes it, carried by the next heat-bath
+    # sweep (su2_mc_up_measured): steps with and without it, alternating,
+    # one bracket per step; the difference is the measurement's cost
+    def step_pair(i):
+        if i % 2 == 0:
+            g.mc_up_measured(q.BC_PERIODIC, res.value, res.value + 8, q.ALGO_HEATBATH, q.UPDATE_LINK)
+        else:
+            g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
+        g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
+
+    sp_ms = bracketed(step_pair, max(2, 2 * (ns // 2)))
+    step_meas_ms, step_plain_ms = float(np.mean(sp_ms[0::2])), float(np.mean(sp_ms[1::2]))
     g.set_params(epsilon=0.5, updates=4)
 
     def metro_sweep(i):
@@ -239,7 +252,7 @@ def gpu_bench(args):
     lib.su2_stream_destroy(stream)
     return dict(total_ms=total_ms, wall=wall, V=V, hb_ms=float(np.mean(hb_ms)), or_ms=float(np.mean(or_ms)),
                 hb_ms_min=float(np.min(hb_ms)), E=float(np.mean(meas[:, 0])), P=det_ok,
-                en_ms=float(np.mean(en_ms)), me_ms=float(np.mean(me_ms[1:] if len(me_ms) > 1 else me_ms)))
+                en_ms=float(np.mean(en_ms)), step_meas_ms=step_meas_ms, step_plain_ms=step_plain_ms, me_ms=float(np.mean(me_ms[1:] if len(me_ms) > 1 else me_ms)))
 
 
 def host_cores():
@@ -892,6 +905,10 @@ def main():
                               "achieved_gbs": ub["metro"] * per_launch_updates / (r["me_ms"] * 1e-3) / 1e9,
                               "frac_of_hbm_peak": ub["metro"] * per_launch_updates / (r["me_ms"] * 1e-3) / 1e9
                               / HBM_PEAK_GBS},
+        "measurement_in_sweep": {"avg_us": (r["step_meas_ms"] - r["step_plain_ms"]) * 1e3,
+                                 "step_with_us": r["step_meas_ms"] * 1e3, "step_without_us": r["step_plain_ms"] * 1e3,
+                                 "is": "the step's measurement as it runs there, carried by the next heat-bath "
+                                       "sweep: a step with it minus a step without (HIP events, alternating)"},
         "en_av_launch": {"avg_us": r["en_ms"] * 1e3,
                          "achieved_gbs": ub["en_av"] * V / (r["en_ms"] * 1e-3) / 1e9,
                          "frac_of_hbm_peak": ub["en_av"] * V / (r["en_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},

@@ -179,9 +179,9 @@ int su2_en_av(const su2_lattice *l, const void *phi, int precision, int pmc, dou
  * measurement and sweeps again right after; a driver that holds a
  * measurement back until its next sweep makes the pair one call.  The
  * measurement rides on the sweep's first launch (y, parity 0), which reads
- * nearly every link the measurement needs before it writes; with a sweep
- * block size other than 256 threads (su2_set_sweep_block) the two run
- * separately.  work as for su2_en_av. */
+ * nearly every link the measurement needs before it writes, and its sum on
+ * the second; with a sweep block size other than 256 threads
+ * (su2_set_sweep_block) the two run separately.  work as for su2_en_av. */
 int su2_mc_up_measured(const su2_lattice *l, void *phi, int precision, int *accept, uint32_t *states, int bc,
                        int algo, int update, double *res, double *res2, void *work, void *stream);
 

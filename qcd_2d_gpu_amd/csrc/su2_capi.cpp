@@ -113,10 +113,14 @@ int sweep(const su2_lattice *l, void *phi, int prec, int *accept, uint32_t *stat
      * t launches add each half-site's count for the whole sweep and the y
      * and x launches count nothing; Metropolis counts in every launch. */
     const int block = g_sweep_block;
+    /* a measurement rides on the sweep with workgroups of SU2_EN_BLOCK
+     * threads: pass 1 in the (y, even) launch, pass 2 in the (y, odd) one */
     double *mw = nullptr;
+    su2_en_final_args fin = {nullptr, 0u, 0.0, nullptr, nullptr};
     if (res) {
         if (block == (int)SU2_EN_BLOCK) {
             mw = (double *)work;
+            fin = {mw, su2_en_av_blocks(g), 1.0 / (2.0 * (double)g.Vh), res, res2};
         } else {
             hipError_t e = su2_launch_en_av(bc == SU2_BC_PMC, prec, phi, g, res, res2, (double *)work, S(stream));
             if (e != hipSuccess) return hip_fail(e, "en_av launch");
@@ -126,12 +130,11 @@ int sweep(const su2_lattice *l, void *phi, int prec, int *accept, uint32_t *stat
         for (int par = 0; par < 2; ++par) {
             const int acc_mode = upd == SU2_K_METRO ? SU2_K_ACC_LAUNCH
                                                     : (mu == SU2_DIR_T ? SU2_K_ACC_SWEEP : SU2_K_ACC_NONE);
-            double *m = (mu == SU2_DIR_Y && par == 0) ? mw : nullptr;
-            hipError_t e =
-                su2_launch_sweep(mu, bc, upd, prec, phi, states, accept, g, p, par, block, acc_mode, S(stream), m);
+            const bool y = mu == SU2_DIR_Y && mw;
+            hipError_t e = su2_launch_sweep(mu, bc, upd, prec, phi, states, accept, g, p, par, block, acc_mode,
+                                            S(stream), y && par == 0 ? mw : nullptr,
+                                            y && par == 1 ? &fin : nullptr);
             if (e != hipSuccess) return hip_fail(e, "sweep launch");
-            if (m && (e = su2_launch_en_av_final(g, m, res, res2, S(stream))) != hipSuccess)
-                return hip_fail(e, "en_av launch");
         }
     return SU2_OK;
 }

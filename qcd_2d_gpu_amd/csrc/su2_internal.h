@@ -143,13 +143,24 @@ static inline uint32_t su2_grid_blocks(uint64_t nthreads, uint32_t block)
     return (uint32_t)nb;
 }
 
+/* en_av pass 2 carried by a sweep launch: the partials, their count (logical
+ * blocks), 1 / V and the result slots it adds into. */
+struct su2_en_final_args {
+    const double *work;
+    uint32_t n;
+    double invV;
+    double *res, *res2;
+};
+
 /* ---- launchers (host), defined in the .hip files ---- */
 /* meas_work non-null: the launch (mu = y, parity 0, block = SU2_EN_BLOCK
  * threads, otherwise hipErrorInvalidValue) also writes the en_av partials of
- * the field it starts from there; su2_launch_en_av_final completes them. */
+ * the field it starts from there.  fin non-null: the launch (mu = y, parity
+ * 1, block = SU2_EN_BLOCK) also runs en_av pass 2 over fin->work, in one
+ * workgroup beside its own (the bits of su2_launch_en_av_final). */
 hipError_t su2_launch_sweep(int mu, int bc, int upd, int f32, void *U, uint32_t *rng, int *accept,
                             const su2_kgeom &g, const su2_kphys &p, int parity, int block, int acc_mode,
-                            hipStream_t st, double *meas_work = nullptr);
+                            hipStream_t st, double *meas_work = nullptr, const su2_en_final_args *fin = nullptr);
 hipError_t su2_launch_setup_rng(uint32_t *rng, const su2_kgeom &g, uint64_t seed, hipStream_t st);
 /* start: 0 reference hot start, 1 cold, 2 reference PCC hot start, 3 Haar,
  * 4 Haar with PCC identity links at y = 0 */

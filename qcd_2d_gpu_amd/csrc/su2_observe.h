@@ -102,12 +102,13 @@ __device__ static inline void su2_block_sum2(double &u, double &v, uint32_t nthr
  * (s0 + s1) + (s2 + s3).  Eight partials are loaded per trip, so a lattice
  * of up to 8 B partials takes one memory round trip; a slot past n reads
  * partial 0 and is not added. */
-__device__ static inline __attribute__((always_inline)) void su2_en_final_partials(const double *__restrict__ work,
-                                                                                uint32_t n, double *te, double *tp)
+__device__ static inline __attribute__((always_inline)) void su2_en_final_partials_at(const double *__restrict__ work,
+                                                                                   uint32_t n, uint32_t t,
+                                                                                   double *te, double *tp)
 {
     const uint32_t B = SU2_EN_FINAL_BLOCK;
     double se[4] = {0.0, 0.0, 0.0, 0.0}, sp[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t i = threadIdx.x; i < n; i += 8 * B) {
+    for (uint32_t i = t; i < n; i += 8 * B) {
         double ve[8], vp[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -125,6 +126,54 @@ __device__ static inline __attribute__((always_inline)) void su2_en_final_partia
     }
     *te = (se[0] + se[1]) + (se[2] + se[3]);
     *tp = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+}
+
+__device__ static inline __attribute__((always_inline)) void su2_en_final_partials(const double *__restrict__ work,
+                                                                                uint32_t n, double *te, double *tp)
+{
+    su2_en_final_partials_at(work, n, threadIdx.x, te, tp);
+}
+
+/* Pass 2 of en_av by a workgroup of SU2_EN_BLOCK threads with the sums and
+ * the order of a SU2_EN_FINAL_BLOCK workgroup (su2_en_final_partials, then
+ * su2_block_sum2), so the bits are the same: thread r stands for threads
+ * r + SU2_EN_BLOCK j, j = 0..3, of the larger workgroup.  Their wavefront
+ * is (r / 64) + 4 j and their lane r % 64, the lane of r in its own
+ * wavefront, so one butterfly per j over the wavefront's lanes forms that
+ * wavefront's total; thread 0 adds the 16 totals in wavefront order and
+ * accumulates sum * invV into res, res2. */
+__device__ static inline void su2_en_final_small(const double *__restrict__ work, uint32_t n, double invV,
+                                                 double *__restrict__ res, double *__restrict__ res2)
+{
+    static_assert(SU2_EN_FINAL_BLOCK == 4 * SU2_EN_BLOCK, "four threads of the larger workgroup per thread");
+    __shared__ double sh[2][16];
+    const uint32_t r = threadIdx.x, lane = r & 63u, w = r >> 6;
+    /* one j at a time: unrolled, the loads of all four would hold 128 VGPRs
+     * and set the register count of the sweep kernel that carries this */
+#pragma nounroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        double u, v;
+        su2_en_final_partials_at(work, n, r + SU2_EN_BLOCK * j, &u, &v);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            u += __shfl_xor(u, off, 64);
+            v += __shfl_xor(v, off, 64);
+        }
+        if (lane == 0) {
+            sh[0][w + 4 * j] = u;
+            sh[1][w + 4 * j] = v;
+        }
+    }
+    __syncthreads();
+    if (r == 0) {
+        double su = 0.0, sv = 0.0;
+        for (int k = 0; k < 16; ++k) {
+            su += sh[0][k];
+            sv += sh[1][k];
+        }
+        *res += su * invV;
+        *res2 += sv * invV;
+    }
 }
 
 #endif /* SU2_OBSERVE_H */

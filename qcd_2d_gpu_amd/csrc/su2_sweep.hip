@@ -24,19 +24,33 @@ namespace {
 /* MEAS (the y launch of parity 0, 256-thread workgroups): the launch also
  * measures the field it starts from and writes the en_av partial pair of its
  * block (su2_observe.h) to meas_work, in the layout of su2_launch_en_av. */
-template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG = false, bool MEAS = false>
+/* FIN: the launch has 8 more workgroups in front of the sweep's; the first
+ * runs en_av pass 2 of the measuring launch before it (su2_en_final_small),
+ * the other seven end at once.  In the sweep that replaces a launch of its
+ * own: the kernel (4.7 us) and its dispatch gap, 8 us per 128^3 step. */
+template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG = false, bool MEAS = false, bool FIN = false>
 __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *__restrict__ U,
                                                                         uint32_t *__restrict__ rng,
                                                                         int *__restrict__ accept, const su2_kgeom &g,
                                                                         const su2_kphys &ph, int parity,
                                                                         uint32_t order, uint32_t strip,
                                                                         uint32_t bpp, uint32_t nthr, int acc_mode,
-                                                                        double *__restrict__ meas_work = nullptr)
+                                                                        double *__restrict__ meas_work,
+                                                                        const su2_en_final_args &fin)
 {
     /* nthr = blockDim.x, passed as an argument: blockDim.x itself is read
      * from the dispatch packet with a vector load whose wait would sit in
      * front of every wavefront's first link and RNG loads */
-    const uint32_t lb = su2_block_of(order, strip, bpp);
+    uint32_t wg = blockIdx.x, nwg = gridDim.x;
+    if constexpr (FIN) {
+        if (wg < 8u) {
+            if (wg == 0u) su2_en_final_small(fin.work, fin.n, fin.invV, fin.res, fin.res2);
+            return;
+        }
+        wg -= 8u;
+        nwg -= 8u;
+    }
+    const uint32_t lb = su2_block_at(wg, nwg, order, strip, bpp);
     const uint32_t h = lb * nthr + threadIdx.x;
     if constexpr (MEAS) {
         su2_owner_plaq m = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
@@ -70,14 +84,14 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
     }
 }
 
-template <int MU, int BC, int UPD, typename LinkT, bool MEAS = false>
+template <int MU, int BC, int UPD, typename LinkT, bool MEAS = false, bool FIN = false>
 __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *__restrict__ rng,
                                                int *__restrict__ accept, su2_kgeom g, su2_kphys ph, int parity,
                                                uint32_t order, uint32_t strip, uint32_t bpp, uint32_t nthr,
-                                               int acc_mode, double *__restrict__ meas_work)
+                                               int acc_mode, double *__restrict__ meas_work, su2_en_final_args fin)
 {
-    sweep_body<MU, BC, UPD, LinkT, false, MEAS>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr, acc_mode,
-                                                meas_work);
+    sweep_body<MU, BC, UPD, LinkT, false, MEAS, FIN>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr,
+                                                     acc_mode, meas_work, fin);
 }
 
 /* FP64-link sweeps run at most 4 waves per SIMD.  More workgroups in flight
@@ -89,14 +103,14 @@ __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *
  * over-relaxation 94) a cap of 5 (100 VGPRs, no spills) timed the same as
  * 4 at 128^3.  FP32 links (half the footprint) are as fast or faster
  * uncapped. */
-template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG, bool MEAS = false>
+template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG, bool MEAS = false, bool FIN = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void
 k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
            int parity, uint32_t order, uint32_t strip, uint32_t bpp, uint32_t nthr, int acc_mode,
-           double *__restrict__ meas_work)
+           double *__restrict__ meas_work, su2_en_final_args fin)
 {
-    sweep_body<MU, BC, UPD, LinkT, STREAM_RNG, MEAS>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr, acc_mode,
-                                                    meas_work);
+    sweep_body<MU, BC, UPD, LinkT, STREAM_RNG, MEAS, FIN>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr,
+                                                         acc_mode, meas_work, fin);
 }
 
 /* Bytes of links and RNG state above which a lattice's launches run from
@@ -105,7 +119,7 @@ k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ 
 
 template <int MU, int BC, int UPD, typename LinkT>
 hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, const su2_kphys &p, int parity,
-                      int block, int acc_mode, double *meas_work, hipStream_t st)
+                      int block, int acc_mode, double *meas_work, const su2_en_final_args *fin, hipStream_t st)
 {
     const uint32_t nb = su2_grid_blocks(g.Vh, (uint32_t)block);
     const bool f64 = sizeof(LinkT) == sizeof(su2_l64);
@@ -119,6 +133,7 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
     const dim3 G(nb), B(block);
     const uint32_t nt = (uint32_t)block;
     LinkT *u = (LinkT *)U;
+    const su2_en_final_args nofin = {nullptr, 0u, 0.0, nullptr, nullptr};
     if constexpr (MU == SU2_DIR_Y) {
         if (meas_work) {
             /* the measuring launch: the host guarantees parity 0 and
@@ -126,16 +141,36 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
             if constexpr (f64 && UPD == SU2_K_HB) {
                 if (dram)
                     hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, true, true>), G, B, 0, st, u, rng, accept, g, p,
-                                       parity, order, strip, bpp, nt, acc_mode, meas_work);
+                                       parity, order, strip, bpp, nt, acc_mode, meas_work, nofin);
                 else
                     hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false, true>), G, B, 0, st, u, rng, accept, g,
-                                       p, parity, order, strip, bpp, nt, acc_mode, meas_work);
+                                       p, parity, order, strip, bpp, nt, acc_mode, meas_work, nofin);
             } else if constexpr (f64) {
                 hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false, true>), G, B, 0, st, u, rng, accept, g, p,
-                                   parity, order, strip, bpp, nt, acc_mode, meas_work);
+                                   parity, order, strip, bpp, nt, acc_mode, meas_work, nofin);
             } else {
                 hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT, true>), G, B, 0, st, u, rng, accept, g, p, parity,
-                                   order, strip, bpp, nt, acc_mode, meas_work);
+                                   order, strip, bpp, nt, acc_mode, meas_work, nofin);
+            }
+            return hipGetLastError();
+        }
+        if (fin) {
+            /* pass 2 of the measuring launch in 8 leading workgroups: the
+             * host guarantees parity 1 and SU2_EN_BLOCK threads */
+            const dim3 GF(nb + 8u);
+            if constexpr (f64 && UPD == SU2_K_HB) {
+                if (dram)
+                    hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, true, false, true>), GF, B, 0, st, u, rng,
+                                       accept, g, p, parity, order, strip, bpp, nt, acc_mode, nullptr, *fin);
+                else
+                    hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false, false, true>), GF, B, 0, st, u, rng,
+                                       accept, g, p, parity, order, strip, bpp, nt, acc_mode, nullptr, *fin);
+            } else if constexpr (f64) {
+                hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false, false, true>), GF, B, 0, st, u, rng, accept,
+                                   g, p, parity, order, strip, bpp, nt, acc_mode, nullptr, *fin);
+            } else {
+                hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT, false, true>), GF, B, 0, st, u, rng, accept, g, p,
+                                   parity, order, strip, bpp, nt, acc_mode, nullptr, *fin);
             }
             return hipGetLastError();
         }
@@ -143,54 +178,56 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
     if constexpr (f64 && UPD == SU2_K_HB) {
         if (dram)
             hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, true>), G, B, 0, st, u, rng, accept, g, p, parity,
-                               order, strip, bpp, nt, acc_mode, nullptr);
+                               order, strip, bpp, nt, acc_mode, nullptr, nofin);
         else
             hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false>), G, B, 0, st, u, rng, accept, g, p, parity,
-                               order, strip, bpp, nt, acc_mode, nullptr);
+                               order, strip, bpp, nt, acc_mode, nullptr, nofin);
     } else if constexpr (f64)
         hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false>), G, B, 0, st, u, rng, accept, g, p, parity, order,
-                           strip, bpp, nt, acc_mode, nullptr);
+                           strip, bpp, nt, acc_mode, nullptr, nofin);
     else
         hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT>), G, B, 0, st, u, rng, accept, g, p, parity, order, strip,
-                           bpp, nt, acc_mode, nullptr);
+                           bpp, nt, acc_mode, nullptr, nofin);
     return hipGetLastError();
 }
 
 template <int BC, int UPD, typename LinkT>
 hipError_t launch_mu(int mu, void *U, uint32_t *rng, int *accept, const su2_kgeom &g, const su2_kphys &p, int parity,
-                     int block, int acc_mode, double *mw, hipStream_t st)
+                     int block, int acc_mode, double *mw, const su2_en_final_args *fin, hipStream_t st)
 {
     switch (mu) {
-    case SU2_DIR_Y: return launch_one<SU2_DIR_Y, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, mw, st);
-    case SU2_DIR_X: return launch_one<SU2_DIR_X, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, mw, st);
-    case SU2_DIR_T: return launch_one<SU2_DIR_T, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, mw, st);
+    case SU2_DIR_Y: return launch_one<SU2_DIR_Y, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
+    case SU2_DIR_X: return launch_one<SU2_DIR_X, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
+    case SU2_DIR_T: return launch_one<SU2_DIR_T, BC, UPD, LinkT>(U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
     }
     return hipErrorInvalidValue;
 }
 
 template <int BC, typename LinkT>
 hipError_t launch_upd(int mu, int upd, void *U, uint32_t *rng, int *accept, const su2_kgeom &g, const su2_kphys &p,
-                      int parity, int block, int acc_mode, double *mw, hipStream_t st)
+                      int parity, int block, int acc_mode, double *mw, const su2_en_final_args *fin,
+                      hipStream_t st)
 {
     switch (upd) {
-    case SU2_K_METRO: return launch_mu<BC, SU2_K_METRO, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, mw, st);
-    case SU2_K_HB: return launch_mu<BC, SU2_K_HB, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, mw, st);
-    case SU2_K_OR: return launch_mu<BC, SU2_K_OR, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, mw, st);
+    case SU2_K_METRO: return launch_mu<BC, SU2_K_METRO, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
+    case SU2_K_HB: return launch_mu<BC, SU2_K_HB, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
+    case SU2_K_OR: return launch_mu<BC, SU2_K_OR, LinkT>(mu, U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
     }
     return hipErrorInvalidValue;
 }
 
 template <typename LinkT>
 hipError_t launch_bc(int mu, int bc, int upd, void *U, uint32_t *rng, int *accept, const su2_kgeom &g,
-                     const su2_kphys &p, int parity, int block, int acc_mode, double *mw, hipStream_t st)
+                     const su2_kphys &p, int parity, int block, int acc_mode, double *mw,
+                     const su2_en_final_args *fin, hipStream_t st)
 {
     switch (bc) {
     case SU2_K_BC_PER:
-        return launch_upd<SU2_K_BC_PER, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, mw, st);
+        return launch_upd<SU2_K_BC_PER, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
     case SU2_K_BC_PCC:
-        return launch_upd<SU2_K_BC_PCC, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, mw, st);
+        return launch_upd<SU2_K_BC_PCC, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
     case SU2_K_BC_PMC:
-        return launch_upd<SU2_K_BC_PMC, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, mw, st);
+        return launch_upd<SU2_K_BC_PMC, LinkT>(mu, upd, U, rng, accept, g, p, parity, block, acc_mode, mw, fin, st);
     }
     return hipErrorInvalidValue;
 }
@@ -199,9 +236,10 @@ hipError_t launch_bc(int mu, int bc, int upd, void *U, uint32_t *rng, int *accep
 
 hipError_t su2_launch_sweep(int mu, int bc, int upd, int f32, void *U, uint32_t *rng, int *accept,
                             const su2_kgeom &g, const su2_kphys &p, int parity, int block, int acc_mode,
-                            hipStream_t st, double *meas_work)
+                            hipStream_t st, double *meas_work, const su2_en_final_args *fin)
 {
     if (meas_work && (mu != SU2_DIR_Y || parity != 0 || block != (int)SU2_EN_BLOCK)) return hipErrorInvalidValue;
-    if (f32) return launch_bc<su2_l32>(mu, bc, upd, U, rng, accept, g, p, parity, block, acc_mode, meas_work, st);
-    return launch_bc<su2_l64>(mu, bc, upd, U, rng, accept, g, p, parity, block, acc_mode, meas_work, st);
+    if (fin && (mu != SU2_DIR_Y || parity != 1 || block != (int)SU2_EN_BLOCK || !fin->work)) return hipErrorInvalidValue;
+    if (f32) return launch_bc<su2_l32>(mu, bc, upd, U, rng, accept, g, p, parity, block, acc_mode, meas_work, fin, st);
+    return launch_bc<su2_l64>(mu, bc, upd, U, rng, accept, g, p, parity, block, acc_mode, meas_work, fin, st);
 }

@@ -103,10 +103,13 @@ template <> struct su2_link_io<su2_l32> {
 enum { ORDER_SLAB = 0, ORDER_STRIPS = 1, ORDER_COLUMNS = 2 };
 
 
-__device__ static inline __attribute__((always_inline)) uint32_t su2_block_of(uint32_t order, uint32_t strip,
+/* Workgroup b of nblk (nblk a multiple of 8) -> block (su2_block_of for a
+ * launch whose first workgroups do other work). */
+__device__ static inline __attribute__((always_inline)) uint32_t su2_block_at(uint32_t b, uint32_t nblk,
+                                                                              uint32_t order, uint32_t strip,
                                                                               uint32_t bpp)
 {
-    const uint32_t b = blockIdx.x, xcd = b & 7u, j = b >> 3, per = gridDim.x >> 3;
+    const uint32_t xcd = b & 7u, j = b >> 3, per = nblk >> 3;
     if (order == ORDER_COLUMNS) {
         const uint32_t W = bpp >> 3; /* blocks of a plane in this XCD's column */
         const uint32_t tl = j / W;
@@ -117,6 +120,12 @@ __device__ static inline __attribute__((always_inline)) uint32_t su2_block_of(ui
     const uint32_t s = j / (T * strip), r = j - s * (T * strip);
     const uint32_t tl = r / strip, c = r - tl * strip;
     return (xcd * T + tl) * bpp + s * strip + c;
+}
+
+__device__ static inline __attribute__((always_inline)) uint32_t su2_block_of(uint32_t order, uint32_t strip,
+                                                                              uint32_t bpp)
+{
+    return su2_block_at(blockIdx.x, gridDim.x, order, strip, bpp);
 }
 
 /* The block order of a launch over the half-sites in blocks of `block`
@@ -416,6 +425,14 @@ __device__ static inline __attribute__((always_inline)) void su2_sweep_site(Link
 {
     typedef su2_link_io<LinkT> io;
 
+    /* the RNG state is loaded ahead of the staple, so that its latency
+     * overlaps the link loads (FP32-link heat-bath at 128^3: -0.8 %,
+     * alternating on one box; Metropolis unchanged) */
+    uint32_t s[6];
+    if (UPD != SU2_K_OR) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * g.Vh + h];
+    }
     su2_q A;
     if (!su2_site_staple<MU, BC, LinkT, MEAS>(U, g, parity, h, &A, bc, meas)) {
         /* a link the boundary does not update (PCC x/t at y = 0): its
@@ -431,9 +448,6 @@ __device__ static inline __attribute__((always_inline)) void su2_sweep_site(Link
         io::store(dst, su2_over_relax(A, u));
         acc = 1;
     } else {
-        uint32_t s[6];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) s[i] = rng[(size_t)i * Vh + h];
         if (UPD == SU2_K_HB) {
             io::store(dst, su2_heat_bath(A, ph.beta, s));
             acc = 1;
