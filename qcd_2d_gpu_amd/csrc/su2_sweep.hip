@@ -102,7 +102,11 @@ __global__ __launch_bounds__(256) void k_sweep(LinkT *__restrict__ U, uint32_t *
  * -6 % and -5 %.  With the fused arithmetic (heat-bath 106 VGPRs,
  * over-relaxation 94) a cap of 5 (100 VGPRs, no spills) timed the same as
  * 4 at 128^3.  FP32 links (half the footprint) are as fast or faster
- * uncapped. */
+ * uncapped.  Over-relaxation, and the heat-bath of a lattice beyond the
+ * Infinity Cache, run at most 3 (k_sweep_w3): alternating on one box,
+ * over-relaxation -1 % at 128^3 and -0.4 % at 256^3, heat-bath -1.5 % at
+ * 256^3 but +5 % at 128^3 (so the cache-resident heat-bath keeps 4; 2 waves
+ * took the 256^3 heat-bath +8 %). */
 template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG, bool MEAS = false, bool FIN = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void
 k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
@@ -111,6 +115,37 @@ k_sweep_w4(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ 
 {
     sweep_body<MU, BC, UPD, LinkT, STREAM_RNG, MEAS, FIN>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr,
                                                          acc_mode, meas_work, fin);
+}
+
+template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG, bool MEAS = false, bool FIN = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void
+k_sweep_w3(LinkT *__restrict__ U, uint32_t *__restrict__ rng, int *__restrict__ accept, su2_kgeom g, su2_kphys ph,
+           int parity, uint32_t order, uint32_t strip, uint32_t bpp, uint32_t nthr, int acc_mode,
+           double *__restrict__ meas_work, su2_en_final_args fin)
+{
+    sweep_body<MU, BC, UPD, LinkT, STREAM_RNG, MEAS, FIN>(U, rng, accept, g, ph, parity, order, strip, bpp, nthr,
+                                                         acc_mode, meas_work, fin);
+}
+
+/* The kernel of one launch: FP32 links uncapped; FP64 links at most 3 waves
+ * per SIMD for over-relaxation and the streaming heat-bath, else 4. */
+template <int MU, int BC, int UPD, typename LinkT, bool STREAM_RNG, bool MEAS, bool FIN>
+hipError_t launch_kernel(dim3 G, dim3 B, hipStream_t st, LinkT *u, uint32_t *rng, int *accept, const su2_kgeom &g,
+                         const su2_kphys &p, int parity, uint32_t order, uint32_t strip, uint32_t bpp, uint32_t nt,
+                         int acc_mode, double *meas_work, const su2_en_final_args &fin)
+{
+    if constexpr (sizeof(LinkT) != sizeof(su2_l64)) {
+        static_assert(!STREAM_RNG, "streaming RNG hints are an FP64-link heat-bath form");
+        hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT, MEAS, FIN>), G, B, 0, st, u, rng, accept, g, p, parity, order,
+                           strip, bpp, nt, acc_mode, meas_work, fin);
+    } else if constexpr (UPD == SU2_K_OR || STREAM_RNG) {
+        hipLaunchKernelGGL((k_sweep_w3<MU, BC, UPD, LinkT, STREAM_RNG, MEAS, FIN>), G, B, 0, st, u, rng, accept, g, p,
+                           parity, order, strip, bpp, nt, acc_mode, meas_work, fin);
+    } else {
+        hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, STREAM_RNG, MEAS, FIN>), G, B, 0, st, u, rng, accept, g, p,
+                           parity, order, strip, bpp, nt, acc_mode, meas_work, fin);
+    }
+    return hipGetLastError();
 }
 
 /* Bytes of links and RNG state above which a lattice's launches run from
@@ -134,61 +169,40 @@ hipError_t launch_one(void *U, uint32_t *rng, int *accept, const su2_kgeom &g, c
     const uint32_t nt = (uint32_t)block;
     LinkT *u = (LinkT *)U;
     const su2_en_final_args nofin = {nullptr, 0u, 0.0, nullptr, nullptr};
+    /* the streaming kernel: heat-bath, FP64 links, beyond the Infinity Cache */
+    constexpr bool srng_ok = sizeof(LinkT) == sizeof(su2_l64) && UPD == SU2_K_HB;
     if constexpr (MU == SU2_DIR_Y) {
         if (meas_work) {
             /* the measuring launch: the host guarantees parity 0 and
              * SU2_EN_BLOCK threads */
-            if constexpr (f64 && UPD == SU2_K_HB) {
+            if constexpr (srng_ok)
                 if (dram)
-                    hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, true, true>), G, B, 0, st, u, rng, accept, g, p,
-                                       parity, order, strip, bpp, nt, acc_mode, meas_work, nofin);
-                else
-                    hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false, true>), G, B, 0, st, u, rng, accept, g,
-                                       p, parity, order, strip, bpp, nt, acc_mode, meas_work, nofin);
-            } else if constexpr (f64) {
-                hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false, true>), G, B, 0, st, u, rng, accept, g, p,
-                                   parity, order, strip, bpp, nt, acc_mode, meas_work, nofin);
-            } else {
-                hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT, true>), G, B, 0, st, u, rng, accept, g, p, parity,
-                                   order, strip, bpp, nt, acc_mode, meas_work, nofin);
-            }
-            return hipGetLastError();
+                    return launch_kernel<MU, BC, UPD, LinkT, true, true, false>(G, B, st, u, rng, accept, g, p, parity,
+                                                                               order, strip, bpp, nt, acc_mode,
+                                                                               meas_work, nofin);
+            return launch_kernel<MU, BC, UPD, LinkT, false, true, false>(G, B, st, u, rng, accept, g, p, parity, order,
+                                                                        strip, bpp, nt, acc_mode, meas_work, nofin);
         }
         if (fin) {
             /* pass 2 of the measuring launch in 8 leading workgroups: the
              * host guarantees parity 1 and SU2_EN_BLOCK threads */
             const dim3 GF(nb + 8u);
-            if constexpr (f64 && UPD == SU2_K_HB) {
+            if constexpr (srng_ok)
                 if (dram)
-                    hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, true, false, true>), GF, B, 0, st, u, rng,
-                                       accept, g, p, parity, order, strip, bpp, nt, acc_mode, nullptr, *fin);
-                else
-                    hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false, false, true>), GF, B, 0, st, u, rng,
-                                       accept, g, p, parity, order, strip, bpp, nt, acc_mode, nullptr, *fin);
-            } else if constexpr (f64) {
-                hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false, false, true>), GF, B, 0, st, u, rng, accept,
-                                   g, p, parity, order, strip, bpp, nt, acc_mode, nullptr, *fin);
-            } else {
-                hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT, false, true>), GF, B, 0, st, u, rng, accept, g, p,
-                                   parity, order, strip, bpp, nt, acc_mode, nullptr, *fin);
-            }
-            return hipGetLastError();
+                    return launch_kernel<MU, BC, UPD, LinkT, true, false, true>(GF, B, st, u, rng, accept, g, p,
+                                                                               parity, order, strip, bpp, nt,
+                                                                               acc_mode, nullptr, *fin);
+            return launch_kernel<MU, BC, UPD, LinkT, false, false, true>(GF, B, st, u, rng, accept, g, p, parity,
+                                                                        order, strip, bpp, nt, acc_mode, nullptr,
+                                                                        *fin);
         }
     }
-    if constexpr (f64 && UPD == SU2_K_HB) {
+    if constexpr (srng_ok)
         if (dram)
-            hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, true>), G, B, 0, st, u, rng, accept, g, p, parity,
-                               order, strip, bpp, nt, acc_mode, nullptr, nofin);
-        else
-            hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false>), G, B, 0, st, u, rng, accept, g, p, parity,
-                               order, strip, bpp, nt, acc_mode, nullptr, nofin);
-    } else if constexpr (f64)
-        hipLaunchKernelGGL((k_sweep_w4<MU, BC, UPD, LinkT, false>), G, B, 0, st, u, rng, accept, g, p, parity, order,
-                           strip, bpp, nt, acc_mode, nullptr, nofin);
-    else
-        hipLaunchKernelGGL((k_sweep<MU, BC, UPD, LinkT>), G, B, 0, st, u, rng, accept, g, p, parity, order, strip,
-                           bpp, nt, acc_mode, nullptr, nofin);
-    return hipGetLastError();
+            return launch_kernel<MU, BC, UPD, LinkT, true, false, false>(G, B, st, u, rng, accept, g, p, parity, order,
+                                                                        strip, bpp, nt, acc_mode, nullptr, nofin);
+    return launch_kernel<MU, BC, UPD, LinkT, false, false, false>(G, B, st, u, rng, accept, g, p, parity, order, strip,
+                                                                 bpp, nt, acc_mode, nullptr, nofin);
 }
 
 template <int BC, int UPD, typename LinkT>
