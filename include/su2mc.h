@@ -387,6 +387,11 @@ int su2_engine_remove(su2_engine *e, int slot);
 /* Times the candidate (block, grid) shapes on the resident jobs and keeps
  * the fastest; the choice is cached per shape class for the process. */
 int su2_engine_autotune(su2_engine *e, int *block, int *grid);
+/* Sets the sweep launches' shape: block 64, 128, 192 or 256 threads, grid
+ * the cap on their workgroups (0 keeps it).  Ticks already enqueued finish
+ * first.  Results are the same bits for every shape; a measurement is
+ * carried by the next tick's first launch only with 256 threads. */
+int su2_engine_set_shape(su2_engine *e, int block, int grid);
 /* Device memory in use / free in the arena. */
 int su2_engine_memory(su2_engine *e, size_t *used, size_t *capacity);
 

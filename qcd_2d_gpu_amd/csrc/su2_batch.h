@@ -42,23 +42,29 @@ struct su2_bhead {
 /* algorithm classes of the resident jobs for the update launches */
 enum { SU2_BALGO_METRO = 0, SU2_BALGO_HB = 1, SU2_BALGO_MIXED = 2 };
 
-/* measure: the y launch of parity 0 also forms the partials of the
- * measurements the lattices carry into this tick (the previous sweep's), for
- * workgroups of SU2_EN_BLOCK threads only. */
+/* measure: the y launch of parity 0 also forms the carried partials of the
+ * lattices with a pending measurement (pend[slot], the previous sweep's),
+ * for workgroups of SU2_EN_BLOCK threads only. */
 hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, int algo_class, const su2_bhead *H,
                              const su2_bdesc *D, const uint32_t *B, uint32_t max_desc, int parity, int or_pass,
-                             uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *ctr = nullptr,
+                             uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend = nullptr,
                              int measure = 0);
 hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B,
                                  uint32_t max_desc, const uint32_t *ctr, int post, uint32_t grid, hipStream_t st);
-/* The end of a measuring tick: with at_end, en_av pass 1 of the lattices
- * that measure at the end of this tick; then pass 2 of those and of the
- * measurements carried into the tick (carry_on: the sweep launches run
- * workgroups of SU2_EN_BLOCK threads).  Pass 2 also advances every sweep
- * counter, so a measuring tick needs no su2_launch_btick_end. */
-hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, uint32_t max_desc,
-                             uint32_t *ctr, uint32_t grid, uint32_t final_grid, int at_end, int carry_on,
-                             hipStream_t st);
+/* en_av pass 1: pre = 0 at the end of a tick, for the lattices that
+ * measure there (not deferred: carry_on is whether the tick's sweep
+ * launches run workgroups of SU2_EN_BLOCK threads), into their partials;
+ * pre = 1 before a tick's sweeps, for the lattices with a pending
+ * measurement, into their carried partials (a tick whose sweep launches
+ * cannot carry them). */
+hipError_t su2_launch_ben_av_partial(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B,
+                                     uint32_t max_desc, const uint32_t *ctr, const uint32_t *pend, uint32_t grid,
+                                     int carry_on, int pre, hipStream_t st);
+/* en_av pass 2 of the pending and the end-of-tick measurements; it sets
+ * the pending flags of the deferred ones and advances every sweep counter,
+ * so a measuring tick needs no su2_launch_btick_end. */
+hipError_t su2_launch_ben_av_final(const su2_bhead *H, const su2_bdesc *D, uint32_t *ctr, uint32_t *pend,
+                                   uint32_t final_grid, int carry_on, hipStream_t st);
 hipError_t su2_launch_btick_end(const su2_bhead *H, const su2_bdesc *D, uint32_t *ctr, hipStream_t st);
 uint32_t su2_en_av_logical_blocks(const su2_kgeom &g);
 

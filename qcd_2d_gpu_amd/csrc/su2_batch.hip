@@ -89,26 +89,24 @@ __device__ static inline bool measure_now(const su2_bdesc &d, uint32_t s)
     return (k % (uint32_t)d.n_cor) == (uint32_t)d.n_cor - 1u;
 }
 
-/* The measurement of sweep s is carried by the first update launch of the
- * lattice's next tick (carried_in), which forms the en_av partials of the
- * field it starts from: no launch changes the field in between, since a
- * reunitarisation before the sweep happens only during thermalisation.  It
- * needs workgroups of SU2_EN_BLOCK threads in the sweep launches
- * (carry_on).  The last measurement of a lattice, and every one without
- * carry_on, runs as en_av at the end of its tick (measure_at_end). */
+/* The measurement of sweep s is deferred to the lattice's next tick
+ * (deferred: its final pass sets the lattice's pending flag) when the sweep
+ * launches run workgroups of SU2_EN_BLOCK threads (carry_on) and the
+ * lattice has a next tick.  There the first update launch forms the en_av
+ * partials of the field it starts from into the carried partials (partc),
+ * or, if that tick's launches do not run SU2_EN_BLOCK threads, an en_av
+ * pass before the sweep does; no launch changes the field in between,
+ * since a reunitarisation before the sweep happens only during
+ * thermalisation.  The tick's final pass adds them and clears the flag.
+ * Every other measurement runs as en_av at the end of its tick. */
 __device__ static inline uint32_t total_sweeps(const su2_bdesc &d)
 {
     return (uint32_t)d.n_therm + (uint32_t)d.n_meas * (uint32_t)d.n_cor;
 }
 
-__device__ static inline bool carried_in(const su2_bdesc &d, uint32_t s, int carry_on)
+__device__ static inline bool deferred(const su2_bdesc &d, uint32_t s, int carry_on)
 {
-    return carry_on && s >= 1u && measure_now(d, s - 1u);
-}
-
-__device__ static inline bool measure_at_end(const su2_bdesc &d, uint32_t s, int carry_on)
-{
-    return measure_now(d, s) && (!carry_on || s + 1u == total_sweeps(d));
+    return carry_on && s + 1u < total_sweeps(d);
 }
 
 __device__ static inline bool normalize_pre(const su2_bdesc &d, uint32_t s)
@@ -183,7 +181,7 @@ __device__ static inline __attribute__((always_inline)) void site_link_measured(
 template <int MU, int OR, int ALGO, typename LinkT, bool MEAS = false>
 __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
                                                 const uint32_t *__restrict__ B, int parity, int or_pass,
-                                                const uint32_t *__restrict__ ctr)
+                                                const uint32_t *__restrict__ pend)
 {
     __shared__ uint32_t s_begin[SU2_BATCH_LDS_DESC];
     const uint32_t n = H->n_active;
@@ -212,7 +210,7 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
             const uint32_t h = lb * blockDim.x + threadIdx.x;
             const int in_range = h < d.g.Vh;
             if constexpr (MEAS) {
-                if (carried_in(d, ctr[d.slot], 1)) {
+                if (pend[d.slot]) {
                     site_link_measured<MU, ALGO, LinkT>(d, parity, h, in_range, q, lb);
                     continue;
                 }
@@ -250,15 +248,18 @@ __global__ __launch_bounds__(256) void k_bnormalize(const su2_bhead *__restrict_
     }
 }
 
-/* en_av pass 1 for the lattices that measure this tick; logical block b of
- * a lattice has the single-lattice layout and bits (su2_en_av_owner_block). */
+/* en_av pass 1; logical block b of a lattice has the single-lattice layout
+ * and bits (su2_en_av_owner_block).  pre = 0: at the end of the tick, for
+ * the lattices that measure there, into part; pre = 1: before the sweeps,
+ * for the lattices with a pending measurement, into partc. */
 template <typename LinkT>
 __device__ static inline __attribute__((always_inline)) void ben_av_partial(const su2_bhead *__restrict__ H,
                                                                             const su2_bdesc *__restrict__ D,
                                                                             const uint32_t *__restrict__ B,
                                                                             uint32_t max_desc,
                                                                             const uint32_t *__restrict__ ctr,
-                                                                            int carry_on)
+                                                                            const uint32_t *__restrict__ pend,
+                                                                            int carry_on, int pre)
 {
     __shared__ uint32_t s_begin[SU2_BATCH_LDS_DESC];
     const uint32_t n = H->n_active;
@@ -269,8 +270,14 @@ __device__ static inline __attribute__((always_inline)) void ben_av_partial(cons
     const bool lds = stage_begins(Bk, n, s_begin);
     for (uint32_t vb = vb0; vb < vend; vb += step) {
         const su2_bdesc &d = D[find_desc(Bk, s_begin, lds, n, vb)];
-        if (!measure_at_end(d, ctr[d.slot], carry_on)) continue;
-        su2_en_av_owner_block<LinkT>((const LinkT *)d.U, d.g, d.bc == SU2_K_BC_PMC, vb - d.en_begin, d.part);
+        if (pre) {
+            if (!pend[d.slot]) continue;
+        } else {
+            const uint32_t s = ctr[d.slot];
+            if (!measure_now(d, s) || deferred(d, s, carry_on)) continue;
+        }
+        su2_en_av_owner_block<LinkT>((const LinkT *)d.U, d.g, d.bc == SU2_K_BC_PMC, vb - d.en_begin,
+                                     pre ? d.partc : d.part);
         __syncthreads(); /* su2_block_sum2's shared array is reused by the next block */
     }
 }
@@ -280,17 +287,19 @@ __device__ static inline __attribute__((always_inline)) void ben_av_partial(cons
  * ensemble.  FP32 links, one thread per owner, would spill under that cap. */
 __global__ __launch_bounds__(2 * SU2_EN_BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_ben_av_partial64(
     const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D, const uint32_t *__restrict__ B,
-    uint32_t max_desc, const uint32_t *__restrict__ ctr, int carry_on)
+    uint32_t max_desc, const uint32_t *__restrict__ ctr, const uint32_t *__restrict__ pend, int carry_on, int pre)
 {
-    ben_av_partial<su2_l64>(H, D, B, max_desc, ctr, carry_on);
+    ben_av_partial<su2_l64>(H, D, B, max_desc, ctr, pend, carry_on, pre);
 }
 
 __global__ __launch_bounds__(SU2_EN_BLOCK) void k_ben_av_partial32(const su2_bhead *__restrict__ H,
                                                                    const su2_bdesc *__restrict__ D,
                                                                    const uint32_t *__restrict__ B, uint32_t max_desc,
-                                                                   const uint32_t *__restrict__ ctr, int carry_on)
+                                                                   const uint32_t *__restrict__ ctr,
+                                                                   const uint32_t *__restrict__ pend, int carry_on,
+                                                                   int pre)
 {
-    ben_av_partial<su2_l32>(H, D, B, max_desc, ctr, carry_on);
+    ben_av_partial<su2_l32>(H, D, B, max_desc, ctr, pend, carry_on, pre);
 }
 
 /* Adds one measurement of lattice d, pass 2 over its partials part, into
@@ -308,24 +317,29 @@ __device__ static inline void add_measurement(const su2_bdesc &d, const double *
     __syncthreads(); /* su2_block_sum2's shared array is reused by the next call */
 }
 
-/* en_av pass 2: workgroup j (1024 threads) adds lattice j's carried
- * measurement (of the sweep before this tick, partials from the tick's
- * first update launch) and its end-of-tick one, each in the single-lattice
- * order, into their measurement slots.  It is the last launch of a
- * measuring tick, so it also ends the tick: every lattice's sweep counter
- * is read once and then advanced by one. */
+/* en_av pass 2: workgroup j (1024 threads) adds lattice j's pending
+ * measurement (of the sweep before this tick, from its carried partials)
+ * and its end-of-tick one, each in the single-lattice order, into their
+ * measurement slots, or defers the end-of-tick one to the next tick.  It
+ * is the last launch of a measuring tick, so it also ends the tick: every
+ * lattice's sweep counter and pending flag are read once and then set. */
 __global__ __launch_bounds__(1024) void k_ben_av_final(const su2_bhead *__restrict__ H,
                                                        const su2_bdesc *__restrict__ D,
-                                                       uint32_t *__restrict__ ctr, int carry_on)
+                                                       uint32_t *__restrict__ ctr, uint32_t *__restrict__ pend,
+                                                       int carry_on)
 {
     const uint32_t n = H->n_active;
     for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
         const su2_bdesc &d = D[j];
-        const uint32_t s = ctr[d.slot];
-        __syncthreads(); /* every wave has read the counter before lane 0 advances it */
-        if (carried_in(d, s, carry_on)) add_measurement(d, d.partc, (s - 1u - (uint32_t)d.n_therm) / (uint32_t)d.n_cor);
-        if (measure_at_end(d, s, carry_on)) add_measurement(d, d.part, (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor);
-        if (threadIdx.x == 0) ctr[d.slot] = s + 1u;
+        const uint32_t s = ctr[d.slot], p = pend[d.slot];
+        __syncthreads(); /* every wave has read them before lane 0 sets them */
+        if (p) add_measurement(d, d.partc, (s - 1u - (uint32_t)d.n_therm) / (uint32_t)d.n_cor);
+        const bool now = measure_now(d, s), later = now && deferred(d, s, carry_on);
+        if (now && !later) add_measurement(d, d.part, (s - (uint32_t)d.n_therm) / (uint32_t)d.n_cor);
+        if (threadIdx.x == 0) {
+            ctr[d.slot] = s + 1u;
+            pend[d.slot] = later ? 1u : 0u;
+        }
     }
 }
 
@@ -340,45 +354,45 @@ __global__ __launch_bounds__(256) void k_btick_end(const su2_bhead *__restrict__
 
 template <int MU, int OR, int ALGO, typename LinkT>
 hipError_t launch_bsweep(const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, int parity, int or_pass,
-                         uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *ctr, int measure)
+                         uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend, int measure)
 {
     if constexpr (MU == SU2_DIR_Y && !OR) {
         if (measure) {
             hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT, true>), dim3(grid), dim3(block), 0, st, H, D, B, parity,
-                               or_pass, ctr);
+                               or_pass, pend);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT>), dim3(grid), dim3(block), 0, st, H, D, B, parity, or_pass,
-                       ctr);
+                       pend);
     return hipGetLastError();
 }
 
 template <int OR, int ALGO, typename LinkT>
 hipError_t launch_bsweep_mu(int mu, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, int parity,
-                            int or_pass, uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *ctr,
+                            int or_pass, uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend,
                             int measure)
 {
     switch (mu) {
-    case 0: return launch_bsweep<0, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, ctr, measure);
-    case 1: return launch_bsweep<1, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, ctr, 0);
-    default: return launch_bsweep<2, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, ctr, 0);
+    case 0: return launch_bsweep<0, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, measure);
+    case 1: return launch_bsweep<1, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, 0);
+    default: return launch_bsweep<2, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, 0);
     }
 }
 
 template <typename LinkT>
 hipError_t launch_bsweep_t(int mu, int over_relax, int algo_class, const su2_bhead *H, const su2_bdesc *D,
                            const uint32_t *B, int parity, int or_pass, uint32_t grid, uint32_t block, hipStream_t st,
-                           const uint32_t *ctr, int measure)
+                           const uint32_t *pend, int measure)
 {
     if (over_relax)
-        return launch_bsweep_mu<1, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, or_pass, grid, block, st, ctr, 0);
+        return launch_bsweep_mu<1, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, or_pass, grid, block, st, pend, 0);
     switch (algo_class) {
     case SU2_BALGO_HB:
-        return launch_bsweep_mu<0, SU2_BALGO_HB, LinkT>(mu, H, D, B, parity, 0, grid, block, st, ctr, measure);
+        return launch_bsweep_mu<0, SU2_BALGO_HB, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure);
     case SU2_BALGO_METRO:
-        return launch_bsweep_mu<0, SU2_BALGO_METRO, LinkT>(mu, H, D, B, parity, 0, grid, block, st, ctr, measure);
-    default: return launch_bsweep_mu<0, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, 0, grid, block, st, ctr, measure);
+        return launch_bsweep_mu<0, SU2_BALGO_METRO, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure);
+    default: return launch_bsweep_mu<0, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure);
     }
 }
 
@@ -386,15 +400,15 @@ hipError_t launch_bsweep_t(int mu, int over_relax, int algo_class, const su2_bhe
 
 hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, int algo_class, const su2_bhead *H,
                              const su2_bdesc *D, const uint32_t *B, uint32_t max_desc, int parity, int or_pass,
-                             uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *ctr, int measure)
+                             uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend, int measure)
 {
     (void)max_desc; /* the sweep range starts are the first array of B */
-    if (measure && (mu != SU2_DIR_Y || parity != 0 || over_relax || block != SU2_EN_BLOCK || !ctr))
+    if (measure && (mu != SU2_DIR_Y || parity != 0 || over_relax || block != SU2_EN_BLOCK || !pend))
         return hipErrorInvalidValue;
     if (f32)
-        return launch_bsweep_t<su2_l32>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st, ctr,
+        return launch_bsweep_t<su2_l32>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st, pend,
                                         measure);
-    return launch_bsweep_t<su2_l64>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st, ctr,
+    return launch_bsweep_t<su2_l64>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st, pend,
                                     measure);
 }
 
@@ -408,21 +422,23 @@ hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D
     return hipGetLastError();
 }
 
-hipError_t su2_launch_ben_av(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, uint32_t max_desc,
-                             uint32_t *ctr, uint32_t grid, uint32_t final_grid, int at_end, int carry_on,
-                             hipStream_t st)
+hipError_t su2_launch_ben_av_partial(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B,
+                                     uint32_t max_desc, const uint32_t *ctr, const uint32_t *pend, uint32_t grid,
+                                     int carry_on, int pre, hipStream_t st)
 {
-    if (at_end) {
-        if (f32)
-            hipLaunchKernelGGL(k_ben_av_partial32, dim3(grid), dim3(su2_en_av_threads(1)), 0, st, H, D, B, max_desc,
-                               ctr, carry_on);
-        else
-            hipLaunchKernelGGL(k_ben_av_partial64, dim3(grid), dim3(su2_en_av_threads(0)), 0, st, H, D, B, max_desc,
-                               ctr, carry_on);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k_ben_av_final, dim3(final_grid), dim3(SU2_EN_FINAL_BLOCK), 0, st, H, D, ctr, carry_on);
+    if (f32)
+        hipLaunchKernelGGL(k_ben_av_partial32, dim3(grid), dim3(su2_en_av_threads(1)), 0, st, H, D, B, max_desc, ctr,
+                           pend, carry_on, pre);
+    else
+        hipLaunchKernelGGL(k_ben_av_partial64, dim3(grid), dim3(su2_en_av_threads(0)), 0, st, H, D, B, max_desc, ctr,
+                           pend, carry_on, pre);
+    return hipGetLastError();
+}
+
+hipError_t su2_launch_ben_av_final(const su2_bhead *H, const su2_bdesc *D, uint32_t *ctr, uint32_t *pend,
+                                   uint32_t final_grid, int carry_on, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ben_av_final, dim3(final_grid), dim3(SU2_EN_FINAL_BLOCK), 0, st, H, D, ctr, pend, carry_on);
     return hipGetLastError();
 }
 

@@ -114,6 +114,7 @@ struct slot_t {
     size_t off = 0;
     su2_kgeom g;
     int total = 0, swept = 0;
+    bool pend = false; /* a measurement deferred to the next tick */
 };
 
 struct tune_key {
@@ -138,12 +139,8 @@ bool h_measure_now(const su2_job &j, int s)
 
 bool h_normalize_pre(const su2_job &j, int s) { return s < j.n_therm && (s + 1) % nevery_of(j) == 0; }
 
-bool h_carried_in(const su2_job &j, int s, bool carry_on) { return carry_on && s >= 1 && h_measure_now(j, s - 1); }
-
-bool h_measure_at_end(const su2_job &j, int s, int total, bool carry_on)
-{
-    return h_measure_now(j, s) && (!carry_on || s + 1 == total);
-}
+/* a measurement after sweep s is deferred to the next tick */
+bool h_deferred(int s, int total, bool carry_on) { return carry_on && s + 1 < total; }
 
 bool h_normalize_post(const su2_job &j, int s)
 {
@@ -154,9 +151,12 @@ bool h_normalize_post(const su2_job &j, int s)
 
 /* which launches a tick needs, and their grids */
 struct tick_plan {
-    /* meas: en_av at the end of the tick; carry: the first update launch
-     * measures for the lattices that carry a measurement into the tick */
-    bool pre = false, post = false, meas = false, carry = false;
+    /* meas: en_av pass 1 at the end of the tick; carry: the first update
+     * launch forms the partials of the pending measurements, carry_pre: a
+     * pass 1 before the sweeps does (a tick that cannot carry them); fin:
+     * en_av pass 2 ends the tick; carry_on: the sweep launches run
+     * SU2_EN_BLOCK threads, so measurements may be deferred */
+    bool pre = false, post = false, meas = false, carry = false, carry_pre = false, fin = false, carry_on = false;
     int n_or = 0;
     int algo = SU2_BALGO_MIXED;
     uint32_t g_sw = 0, g_nm = 0, g_en = 0, g_fin = 0; /* workgroups per launch kind */
@@ -167,7 +167,11 @@ struct tick_plan {
             while ((1u << l) < v) ++l;
             return l;
         };
-        const uint64_t flags = ((((uint64_t)(n_or * 3 + algo) * 2 + pre) * 2 + post) * 2 + meas) * 2 + carry;
+        const uint64_t flags =
+            ((((((((uint64_t)(n_or * 3 + algo) * 2 + pre) * 2 + post) * 2 + meas) * 2 + carry) * 2 + carry_pre) * 2 +
+              fin) *
+                 2 +
+             carry_on);
         return (flags << 24) | (lg(g_sw) << 18) | (lg(g_nm) << 12) | (lg(g_en) << 6) | lg(g_fin);
     }
 };
@@ -196,6 +200,7 @@ struct su2_engine {
     su2_bhead *dH = nullptr;
     su2_bdesc *dD = nullptr;
     uint32_t *dctr = nullptr;
+    uint32_t *dpend = nullptr; /* per slot: a measurement deferred to the next tick */
     uint32_t *dB = nullptr; /* range starts: sweep, en_av, normalise; max_jobs each */
     su2_bhead *hH = nullptr;
     su2_bdesc *hD = nullptr;
@@ -305,16 +310,21 @@ struct su2_engine {
     {
         tick_plan p;
         bool hb = false, metro = false;
-        const bool carry_on = block == (int)SU2_EN_BLOCK;
+        p.carry_on = block == (int)SU2_EN_BLOCK;
+        bool pend = false;
         for (const auto &s : slots) {
             if (!s.used || s.done) continue;
             p.pre |= h_normalize_pre(s.job, s.swept);
             p.post |= h_normalize_post(s.job, s.swept);
-            p.meas |= h_measure_at_end(s.job, s.swept, s.total, carry_on);
-            p.carry |= h_carried_in(s.job, s.swept, carry_on);
+            const bool now = h_measure_now(s.job, s.swept);
+            p.meas |= now && !h_deferred(s.swept, s.total, p.carry_on);
+            p.fin |= now || s.pend;
+            pend |= s.pend;
             p.n_or = std::max(p.n_or, s.job.or_sweeps);
             (s.job.algo == SU2_ALGO_HEATBATH ? hb : metro) = true;
         }
+        p.carry = pend && p.carry_on;
+        p.carry_pre = pend && !p.carry_on;
         p.algo = hb && metro ? SU2_BALGO_MIXED : (hb ? SU2_BALGO_HB : SU2_BALGO_METRO);
         p.g_sw = fit_grid(hH->sw_blocks, (uint32_t)grid);
         p.g_nm = fit_grid(hH->nm_blocks, (uint32_t)aux_grid);
@@ -328,19 +338,22 @@ struct su2_engine {
         const int f32 = cfg.precision == SU2_PREC_F32;
         const uint32_t mj = (uint32_t)cfg.max_jobs;
         hipError_t e = hipSuccess;
-        if (p.pre) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 0, p.g_nm, st);
+        /* the pending measurements, of the field as the last tick left it */
+        if (p.carry_pre) e = su2_launch_ben_av_partial(f32, dH, dD, dB, mj, dctr, dpend, p.g_en, p.carry_on, 1, st);
+        if (p.pre && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 0, p.g_nm, st);
         for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
             for (int par = 0; par < 2 && e == hipSuccess; ++par)
-                e = su2_launch_bsweep(f32, mu, 0, p.algo, dH, dD, dB, mj, par, 0, p.g_sw, block, st, dctr,
+                e = su2_launch_bsweep(f32, mu, 0, p.algo, dH, dD, dB, mj, par, 0, p.g_sw, block, st, dpend,
                                       p.carry && mu == 0 && par == 0);
         for (int pass = 0; pass < p.n_or && e == hipSuccess; ++pass)
             for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
                 for (int par = 0; par < 2 && e == hipSuccess; ++par)
                     e = su2_launch_bsweep(f32, mu, 1, p.algo, dH, dD, dB, mj, par, pass, p.g_sw, block, st);
         if (p.post && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 1, p.g_nm, st);
-        if ((p.meas || p.carry) && e == hipSuccess)
-            e = su2_launch_ben_av(f32, dH, dD, dB, mj, dctr, p.g_en, p.g_fin, p.meas, block == (int)SU2_EN_BLOCK, st);
-        if (!p.meas && !p.carry && e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
+        if (p.meas && e == hipSuccess)
+            e = su2_launch_ben_av_partial(f32, dH, dD, dB, mj, dctr, dpend, p.g_en, p.carry_on, 0, st);
+        if (p.fin && e == hipSuccess) e = su2_launch_ben_av_final(dH, dD, dctr, dpend, p.g_fin, p.carry_on, st);
+        if (!p.fin && e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
         if (e != hipSuccess) return su2_set_error(SU2_EHIP, "tick launch: %s", hipGetErrorString(e));
         return SU2_OK;
     }
@@ -405,11 +418,14 @@ struct su2_engine {
             int rc = enqueue_tick(p);
             if (rc) return rc;
         }
-        for (auto &s : slots)
-            if (s.used && !s.done && ++s.swept == s.total) {
+        for (auto &s : slots) {
+            if (!s.used || s.done) continue;
+            s.pend = h_measure_now(s.job, s.swept) && h_deferred(s.swept, s.total, p.carry_on);
+            if (++s.swept == s.total) {
                 s.done = true;
                 dirty = true;
             }
+        }
         return SU2_OK;
     }
 };
@@ -471,13 +487,16 @@ int su2_engine_create(const su2_engine_config *cfg, su2_engine **out)
     if (hipMalloc((void **)&e->dH, sizeof(su2_bhead)) != hipSuccess ||
         hipMalloc((void **)&e->dD, sizeof(su2_bdesc) * e->cfg.max_jobs) != hipSuccess ||
         hipMalloc((void **)&e->dctr, sizeof(uint32_t) * e->cfg.max_jobs) != hipSuccess ||
+        hipMalloc((void **)&e->dpend, sizeof(uint32_t) * e->cfg.max_jobs) != hipSuccess ||
         hipMalloc((void **)&e->dB, 3 * sizeof(uint32_t) * e->cfg.max_jobs) != hipSuccess ||
         hipHostMalloc((void **)&e->hH, sizeof(su2_bhead), 0) != hipSuccess ||
         hipHostMalloc((void **)&e->hD, sizeof(su2_bdesc) * e->cfg.max_jobs, 0) != hipSuccess ||
         hipHostMalloc((void **)&e->hB, 3 * sizeof(uint32_t) * e->cfg.max_jobs, 0) != hipSuccess ||
         hipEventCreateWithFlags(&e->upload_done, hipEventDisableTiming) != hipSuccess)
         return bail(su2_set_error(SU2_ENOMEM, "engine tables"));
-    if (hipMemsetAsync(e->dctr, 0, sizeof(uint32_t) * e->cfg.max_jobs, e->st) != hipSuccess) return bail(SU2_EHIP);
+    if (hipMemsetAsync(e->dctr, 0, sizeof(uint32_t) * e->cfg.max_jobs, e->st) != hipSuccess ||
+        hipMemsetAsync(e->dpend, 0, sizeof(uint32_t) * e->cfg.max_jobs, e->st) != hipSuccess)
+        return bail(SU2_EHIP);
     rc = e->rebuild();
     if (rc) return bail(rc);
     *out = e;
@@ -495,6 +514,7 @@ int su2_engine_destroy(su2_engine *e)
     if (e->hD) hipHostFree(e->hD);
     if (e->hH) hipHostFree(e->hH);
     if (e->dctr) hipFree(e->dctr);
+    if (e->dpend) hipFree(e->dpend);
     if (e->dD) hipFree(e->dD);
     if (e->dH) hipFree(e->dH);
     if (e->arena) hipFree(e->arena);
@@ -528,6 +548,7 @@ int su2_engine_add(su2_engine *e, const su2_job *job, int *slot)
     s.g = su2_make_kgeom(job->Lt, job->Lx, job->Ly);
     s.total = job->n_therm + job->n_meas * job->n_cor;
     s.swept = 0;
+    s.pend = false;
     char *base = e->arena + off;
     const int f32 = e->cfg.precision == SU2_PREC_F32;
     /* setup_kernel, the start of kernel.cu:85-136, zeroed counters/series */
@@ -541,6 +562,7 @@ int su2_engine_add(su2_engine *e, const su2_job *job, int *slot)
      * launches write only blocks that hold half-sites) */
     if (he == hipSuccess) he = hipMemsetAsync(base + lay.partc, 0, lay.total - lay.partc, e->st);
     if (he == hipSuccess) he = hipMemsetAsync(e->dctr + id, 0, sizeof(uint32_t), e->st);
+    if (he == hipSuccess) he = hipMemsetAsync(e->dpend + id, 0, sizeof(uint32_t), e->st);
     if (he != hipSuccess) {
         e->release(off, lay.total);
         s.used = false;
@@ -657,6 +679,20 @@ int su2_engine_memory(su2_engine *e, size_t *used, size_t *capacity)
     if (!e) return su2_set_error(SU2_EINVAL, "NULL");
     if (used) *used = e->used;
     if (capacity) *capacity = e->cap;
+    return SU2_OK;
+}
+
+int su2_engine_set_shape(su2_engine *e, int block, int grid)
+{
+    if (!e) return su2_set_error(SU2_EINVAL, "NULL");
+    if (block < 64 || block > 256 || block % 64) return su2_set_error(SU2_EINVAL, "block must be 64, 128, 192 or 256");
+    if (grid < 0) return su2_set_error(SU2_EINVAL, "grid must be >= 0");
+    /* ticks enqueued before may still run their graphs */
+    ENG_HIP(hipStreamSynchronize(e->st));
+    e->block = block;
+    if (grid) e->grid = grid;
+    e->dirty = true;
+    e->drop_graph();
     return SU2_OK;
 }
 

@@ -57,6 +57,7 @@ ENGINE_PROTOTYPES = {
     "su2_engine_read": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(C.c_double), _P, _P]),
     "su2_engine_remove": (C.c_int, [_P, C.c_int]),
     "su2_engine_autotune": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "su2_engine_set_shape": (C.c_int, [_P, C.c_int, C.c_int]),
     "su2_engine_memory": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "su2_server_create": (C.c_int, [C.POINTER(Su2EngineConfig), C.POINTER(C.c_void_p)]),
     "su2_server_destroy": (C.c_int, [_P]),
@@ -165,6 +166,10 @@ class Engine:
         b, g = C.c_int(), C.c_int()
         L.check(self.lib.su2_engine_autotune(self.h, C.byref(b), C.byref(g)), "engine_autotune")
         return b.value, g.value
+
+    def set_shape(self, block: int, grid: int = 0) -> None:
+        """Sweep launches of `block` threads (64..256), grid cap `grid` (0 keeps it)."""
+        L.check(self.lib.su2_engine_set_shape(self.h, block, grid), "engine_set_shape")
 
     def memory(self) -> tuple[int, int]:
         u, c = C.c_size_t(), C.c_size_t()

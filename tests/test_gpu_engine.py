@@ -130,6 +130,31 @@ def test_batch_without_carried_measurement(gpu, prec):
         eng.close()
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_shape_changes_with_measurements_pending(gpu, prec):
+    """The sweep launches' workgroup size changed between ticks, 256 -> 128
+    -> 256 -> 64: measurements deferred to the next tick's first launch
+    (possible only at 256) are taken by an en_av pass before that tick's
+    sweeps, and none is taken twice; every job still gives the bits it gives
+    alone."""
+    q = gpu
+    _say(f"[engine] shape changes mid-run, prec={prec}")
+    jobs = mixed_jobs(q, 8, seed0=2700 + 10 * prec)
+    for j in jobs:
+        j.n_meas, j.n_cor = 6, 1  # a measurement after every sweep past thermalisation
+    eng = q.Engine(precision=prec, max_jobs=8, budget_bytes=64 << 20, use_graph=True)
+    try:
+        slots = [eng.add(j) for j in jobs]
+        for block in (128, 256, 64, 256):
+            eng.tick(2)
+            eng.set_shape(block)
+        eng.run()
+        for s, j in zip(slots, jobs):
+            check_same(q, eng, s, j, prec, f"shape changes slot {s} {j}")
+    finally:
+        eng.close()
+
+
 def test_jobs_join_and_leave(gpu):
     """Continuous batching on the engine: jobs added while others run, one
     removed unfinished, results of the rest unchanged."""
