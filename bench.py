@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--roofline-sweeps", type=int, default=20)
     ap.add_argument("--large-L", type=int, default=256, help="extent of the DRAM-bound timing (0: skip)")
+    ap.add_argument("--latency-floor", type=int, default=1,
+                    help="phase-separation probe (launch vs cooperative grid barrier) and small-lattice launches "
+                         "(0: skip; profiled runs skip it)")
     ap.add_argument("--shard-shares", type=int, default=1,
                     help="with the --large-L timing, time the launches on one GPU's share of a t-slab "
                          "decomposition over 8 GPUs (0: skip)")
@@ -303,9 +306,11 @@ def cpu_baseline(args, V_gpu):
     return out
 
 
-def lattice_launch_bench(Lt, Lx, Ly, reps=5):
+def lattice_launch_bench(Lt, Lx, Ly, reps=5, therm=2):
     """Heat-bath and over-relaxation launches of one Lt x Lx x Ly lattice
-    (FP64 links, periodic, beta = 5), after two warm-up sweeps: the six
+    (FP64 links, periodic, beta = 5), after `therm` heat-bath +
+    over-relaxation sweeps from the Haar-random start (reunitarised every
+    100 and at the end, as the thermalisation of kernel.cu:161-179): the six
     launches of a sweep bracketed by one pair of HIP events on the launch
     stream, per launch = bracket / 6."""
     import qcd_2d_gpu_amd as q
@@ -317,10 +322,13 @@ def lattice_launch_bench(Lt, Lx, Ly, reps=5):
     g = q.Lattice(Lt, Lx, Ly, 5.0, precision=q.PREC_F64, stream=stream)
     g.setup_kernel(7)
     g.haar_init()
-    for _ in range(2):
+    for i in range(1, therm + 1):
+        if i % 100 == 0:
+            g.normalize()
         g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_LINK)
         g.mc_up_per(q.ALGO_HEATBATH, q.UPDATE_OVERRELAX)
-    out = {"shape": f"{Lt}x{Lx}x{Ly}", "sites": g.V}
+    g.normalize()
+    out = {"shape": f"{Lt}x{Lx}x{Ly}", "sites": g.V, "thermalisation_sweeps": therm}
     for upd, name, nbytes in ((q.UPDATE_LINK, "heatbath", BYTES_HB), (q.UPDATE_OVERRELAX, "overrelax", BYTES_OR)):
         evs = []
         for _ in range(reps):
@@ -346,10 +354,41 @@ def lattice_launch_bench(Lt, Lx, Ly, reps=5):
     return out
 
 
+def latency_floor_bench():
+    """What a phase of a small lattice's sweep costs: su2_phase_probe (one
+    phase = every thread writes 8 doubles of its own workgroup's, then reads
+    8 of the next workgroup's) separated by back-to-back kernel launches or
+    by a grid-wide barrier in one cooperative launch, for 8, 32 and 128
+    workgroups of 256; beside it the heat-bath and over-relaxation launches
+    of 16^3 and 32^3 lattices (2048 and 16384 updates per launch)."""
+    import qcd_2d_gpu_amd as q
+    from qcd_2d_gpu_amd import _lib as L
+
+    lib = q.load_library()
+    stream = C.c_void_p()
+    L.check(lib.su2_stream_create(C.byref(stream)), "stream_create")
+    words = 8
+    out = {"phase_separation": []}
+    for blocks in (8, 32, 128):
+        buf = q.DeviceBuffer(blocks * 256 * words * 8)
+        buf.zero(stream)
+        lu, bu = C.c_double(), C.c_double()
+        L.check(lib.su2_phase_probe(buf.ptr, blocks, words, 200, stream, C.byref(lu), C.byref(bu)), "phase_probe")
+        buf.free()
+        out["phase_separation"].append({"workgroups": blocks, "launch_us": lu.value, "grid_barrier_us": bu.value})
+    lib.su2_stream_destroy(stream)
+    for n in (16, 32):
+        r = lattice_launch_bench(n, n, n, reps=20, therm=20)
+        out[f"sweep_launch_{n}cube_us"] = {"heatbath": r["heatbath"]["avg_launch_us"],
+                                           "overrelax": r["overrelax"]["avg_launch_us"]}
+    return out
+
+
 def large_lattice_bench(args):
     """L^3 launches (256^3: 1.6 GB of FP64 links, far beyond the 256 MB
-    Infinity Cache, so DRAM-bound)."""
-    out = lattice_launch_bench(args.large_L, args.large_L, args.large_L)
+    Infinity Cache, so DRAM-bound), after the configuration's 200
+    thermalisation sweeps (about 1 s at 256^3)."""
+    out = lattice_launch_bench(args.large_L, args.large_L, args.large_L, reps=10, therm=args.therm)
     out["L"] = args.large_L
     return out
 
@@ -536,8 +575,11 @@ def serving_bench(args, burst=False):
         arrivals[:] = 0.0
     jobs = serving_jobs(q, n, rng)
     srv = q.Server(precision=q.PREC_F64, max_jobs=256, budget_bytes=200 * 10 ** 9, use_graph=True)
-    # warm the server's kernels and graph with one small request
-    srv.submit(q.Job(8, 8, 8, 3.0, n_therm=2, n_meas=2))
+    # warm the server's kernels and the graphs of every tick plan the stream
+    # meets (thermalisation, its last sweep, measuring sweeps, the
+    # reunitarisation every 100 sweeps) with one small request of the same
+    # schedule, as a server is warmed before it takes traffic
+    srv.submit(q.Job(16, 16, 16, 3.0, algo=q.ALGO_HEATBATH, start=q.START_HOT, seed=1, n_therm=50, n_meas=200))
     srv.drain()
     srv.poll()
     base = srv.now()
@@ -845,6 +887,8 @@ def main():
                 d["frac_of_stream_ceiling"] = d["algorithmic_gbs"] / ceil["dram_reads4"]
     if args.large_L and args.shard_shares:
         section("shard_share_fp64", shard_share_bench)
+    if args.latency_floor:
+        section("latency_floor", latency_floor_bench)
     if args.small_configs and args.cpu_steps != 0:
         section("small_configs", lambda: small_config_bench(args))
     if args.batch_jobs:

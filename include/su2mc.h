@@ -234,6 +234,18 @@ int su2_device_copy(void *dst, const void *src, size_t bytes, void *stream);
  * of 256 (0: one item per thread per unroll, at most 2^20). */
 int su2_stream_probe(void *dst, const void *src, size_t bytes, int reads, int unroll, int nontemporal, int blocks,
                      void *stream);
+/* Extension (no counterpart in the reference): what separating two
+ * dependent phases of a sweep costs on this device.  A phase is every thread
+ * of `blocks` workgroups of 256 adding 1 to `words` doubles of its own and
+ * then reading `words` doubles of the next workgroup, as a (direction,
+ * parity) launch depends on the one before it.  The halves are separated
+ * once by kernel boundaries (back-to-back launches on `stream`) and once by
+ * a grid-wide barrier inside one cooperative launch; *launch_us and
+ * *barrier_us are the mean time per separation over `phases` phases.  buf
+ * holds at least blocks * 256 * words doubles; blocks must all fit on the
+ * device at once.  Synchronous. */
+int su2_phase_probe(double *buf, int blocks, int words, int phases, void *stream, double *launch_us,
+                    double *barrier_us);
 int su2_stream_create(void **stream);
 int su2_stream_destroy(void *stream);
 int su2_stream_synchronize(void *stream);

@@ -25,7 +25,8 @@ EXE = os.path.join(PKG, "su2mc")
 ARCH = os.environ.get("SU2MC_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-SOURCES = ["su2_sweep.hip", "su2_misc.hip", "su2_batch.hip", "su2_capi.cpp", "su2_handle.cpp", "su2_engine.cpp"]
+SOURCES = ["su2_sweep.hip", "su2_misc.hip", "su2_batch.hip", "su2_probe.hip", "su2_capi.cpp", "su2_handle.cpp",
+           "su2_engine.cpp"]
 C_SOURCES = ["su2_driver.c"]  # host driver in C, on the C ABI
 CC = os.environ.get("CC", shutil.which("gcc") or "cc")
 CFLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-Wall", "-I", INCLUDE, "-I", CSRC]

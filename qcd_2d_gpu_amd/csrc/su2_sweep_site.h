@@ -86,10 +86,12 @@ template <> struct su2_link_io<su2_l32> {
  * there the time is set by the DRAM access pattern itself (a copy of the
  * kernel with its arithmetic removed takes 99 % of its time).  FP32 links
  * (half the footprint) gain nothing from it and keep the slab order.
- * Strips are 16 blocks wide where the plane's block count allows it, else
- * 8: at 128^3 (32 blocks per plane) 16 against 8 gave heat-bath -4 %, step
- * -2 %; 4 and 32 were slower than 8, and at 256^3 8 was 5 % slower than
- * 16.
+ * Strips are 32 blocks wide on planes of 64 blocks or more, else 16 where
+ * the plane's block count allows it, else 8: at 128^3 (32 blocks per plane)
+ * 16 against 8 gave heat-bath -4 %, step -2 %, and 4 and 32 were slower
+ * than 8; at 256^3 (128 blocks per plane) 8 was 5 % slower than 16, and 32
+ * against 16 gave over-relaxation -3.3 % (380 -> 368 us) with the heat-bath
+ * unchanged (alternating runs on one box; 64 and 128 were 3-4 % slower).
  *
  * ORDER_COLUMNS (thin lattices, Lt <= 32, with bpp a multiple of 8): a slab
  * of Lt/8 <= 4 planes would send half or more of an XCD's t-neighbour reads
@@ -138,7 +140,7 @@ static inline void su2_pick_block_order(const su2_kgeom &g, uint32_t block, bool
     *order = ORDER_SLAB;
     *strip = 0;
     *bpp = g.plane / block;
-    const uint32_t w = (*bpp % 16u == 0) ? 16u : 8u;
+    const uint32_t w = (*bpp >= 64u && *bpp % 32u == 0) ? 32u : (*bpp % 16u == 0) ? 16u : 8u;
     const bool whole = g.plane % block == 0 && nb * block == g.Vh;
     if (f64 && whole && g.Lt <= 32u && *bpp % 8u == 0) {
         *order = ORDER_COLUMNS;

@@ -144,12 +144,14 @@ def test_thin_lattices_column_block_order(gpu, oracle_mod, shape, bc):
     run_pair(gpu, oracle_mod, "columns", *shape, 4.0, bc, 1, 0, 31, 2, updates=(0, 1))
 
 
-@pytest.mark.parametrize("shape,bc", [((64, 64, 64), 0), ((48, 128, 64), 2), ((40, 64, 96), 1)])
+@pytest.mark.parametrize("shape,bc", [((64, 64, 64), 0), ((48, 128, 64), 2), ((40, 64, 96), 1),
+                                      ((40, 256, 128), 0)])
 def test_strip_and_slab_block_orders(gpu, oracle_mod, shape, bc):
     """FP64 lattices of more than 32 t-planes: planes of 8 blocks walk
     8-block x-strips (64^3), planes of 16 blocks 16-block strips
     (48x128x64, like 128^3's 32), planes of 12 blocks the slab order
-    (40x64x96); link updates and over-relaxation."""
+    (40x64x96), planes of 64 blocks 32-block strips (40x256x128, like
+    256^3's 128); link updates and over-relaxation."""
     run_pair(gpu, oracle_mod, "strips", *shape, 4.5, bc, 1, 0, 37, 2, updates=(0, 1))
 
 
