@@ -560,10 +560,14 @@ def serving_jobs(q, n, rng):
                                                    rng.uniform(2.0, 8.0, n), rng.integers(0, 3, n)))]
 
 
-def serving_bench(args, burst=False):
+def serving_bench(args, burst=False, use_graph=False):
     """The serving workload into the continuous-batching server (200 GB
     budget): a Poisson stream at --serve-rate, or with burst=True all
-    --serve-burst requests at once (the server's capacity)."""
+    --serve-burst requests at once (the server's capacity).  The server
+    enqueues each tick's batched launches on its stream (use_graph=False):
+    replaying them as a captured hipGraph made a lone job's tick 4-11 us
+    slower on MI355X (38-58 us per tick at 16^3-64^3 without), and the served
+    p50 about 8 % higher; use_graph=True measures that form."""
     import threading
 
     import qcd_2d_gpu_amd as q
@@ -574,7 +578,7 @@ def serving_bench(args, burst=False):
     if burst:
         arrivals[:] = 0.0
     jobs = serving_jobs(q, n, rng)
-    srv = q.Server(precision=q.PREC_F64, max_jobs=256, budget_bytes=200 * 10 ** 9, use_graph=True)
+    srv = q.Server(precision=q.PREC_F64, max_jobs=256, budget_bytes=200 * 10 ** 9, use_graph=use_graph)
     # warm the server's kernels and the graphs of every tick plan the stream
     # meets (thermalisation, its last sweep, measuring sweeps, the
     # reunitarisation every 100 sweeps) with one small request of the same
@@ -609,6 +613,7 @@ def serving_bench(args, burst=False):
         out["arrival"] = "all at once"
     else:
         out["offered_rate_per_s"] = args.serve_rate
+    out["ticks"] = "replayed hipGraphs" if use_graph else "batched launches enqueued on the server's stream"
     return out
 
 
@@ -900,6 +905,7 @@ def main():
             extra["batched_ensemble_fp32"]["plaquette_vs_fp64"] = fp32_ensemble_error(p64, p32)
     if args.serve_requests:
         section("serving", lambda: serving_bench(args))
+        section("serving_graph_replay", lambda: serving_bench(args, use_graph=True))
     if args.serve_requests and args.serve_burst:
         section("serving_burst", lambda: serving_bench(args, burst=True))
     if args.fp32_meas:

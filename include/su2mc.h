@@ -366,7 +366,9 @@ typedef struct su2_engine_config {
     int max_jobs;        /* resident jobs (0: 256) */
     int max_or;          /* over-relaxation passes per tick (0: 1) */
     size_t budget_bytes; /* device memory for job data (0: 1 GiB; at most 90 % of the free memory) */
-    int use_graph;       /* replay ticks from a captured hipGraph */
+    int use_graph;       /* replay ticks from a captured hipGraph (measured 4-11 us per tick slower than the
+                            launches enqueued one by one for lone 16^3-64^3 jobs, 1-4 % on a 64-lattice
+                            ensemble, on MI355X) */
     int block;           /* sweep workgroup size: 64, 128, 192 or 256 (0: 256; su2_engine_autotune may change it) */
     int grid;            /* most workgroups of a sweep launch (0: 16 per CU; su2_engine_autotune may change it) */
 } su2_engine_config;

@@ -216,10 +216,12 @@ class Result:
 
 class Server:
     """Continuous-batching server: submit jobs any time; results come back
-    as they finish."""
+    as they finish.  Ticks are enqueued as launches on the server's stream
+    by default: replaying them from captured hipGraphs (use_graph=True)
+    measured 4-11 us slower per tick for lone 16^3-64^3 jobs on MI355X."""
 
     def __init__(self, precision: int = L.PREC_F64, max_jobs: int = 256, max_or: int = 1,
-                 budget_bytes: int = 1 << 30, use_graph: bool = True, block: int = 0, grid: int = 0,
+                 budget_bytes: int = 1 << 30, use_graph: bool = False, block: int = 0, grid: int = 0,
                  device: int = 0):
         self.lib = L.load()
         self.h = C.c_void_p()
