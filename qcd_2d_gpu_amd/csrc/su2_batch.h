@@ -42,13 +42,24 @@ struct su2_bhead {
 /* algorithm classes of the resident jobs for the update launches */
 enum { SU2_BALGO_METRO = 0, SU2_BALGO_HB = 1, SU2_BALGO_MIXED = 2 };
 
+/* The end of a tick carried by its last sweep launch: lead leading
+ * workgroups (a multiple of 8) add the pending measurements and advance the
+ * sweep counters and pending flags, the work of su2_launch_ben_av_final when
+ * no lattice measures at the end of the tick. */
+struct su2_btick_end {
+    uint32_t *ctr, *pend;
+    uint32_t lead;
+    int carry_on;
+};
 /* measure: the y launch of parity 0 also forms the carried partials of the
  * lattices with a pending measurement (pend[slot], the previous sweep's),
- * for workgroups of SU2_EN_BLOCK threads only. */
+ * for workgroups of SU2_EN_BLOCK threads only.  end: the launch (mu = t,
+ * parity 1, SU2_EN_BLOCK threads, the tick's last sweep launch) also ends
+ * the tick. */
 hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, int algo_class, const su2_bhead *H,
                              const su2_bdesc *D, const uint32_t *B, uint32_t max_desc, int parity, int or_pass,
                              uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend = nullptr,
-                             int measure = 0);
+                             int measure = 0, const su2_btick_end *end = nullptr);
 hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B,
                                  uint32_t max_desc, const uint32_t *ctr, int post, uint32_t grid, hipStream_t st);
 /* en_av pass 1: pre = 0 at the end of a tick, for the lattices that

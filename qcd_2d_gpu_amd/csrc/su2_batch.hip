@@ -37,19 +37,24 @@ namespace {
  * walks.  Workgroups are dispatched to the 8 XCDs round-robin by blockIdx,
  * so each XCD gets one contiguous eighth of the virtual blocks: tiles of a
  * lattice that share neighbour links meet in the same L2. */
-__device__ static inline void xcd_walk(uint32_t nblk, uint32_t *vb0, uint32_t *vend, uint32_t *step)
+__device__ static inline void xcd_walk_at(uint32_t wg, uint32_t G, uint32_t nblk, uint32_t *vb0, uint32_t *vend,
+                                          uint32_t *step)
 {
-    const uint32_t G = gridDim.x;
     if (G & 7u) {
-        *vb0 = blockIdx.x;
+        *vb0 = wg;
         *vend = nblk;
         *step = G;
         return;
     }
-    const uint32_t per = (nblk + 7u) >> 3, x = blockIdx.x & 7u;
-    *vb0 = x * per + (blockIdx.x >> 3);
+    const uint32_t per = (nblk + 7u) >> 3, x = wg & 7u;
+    *vb0 = x * per + (wg >> 3);
     *vend = min(nblk, (x + 1u) * per);
     *step = G >> 3;
+}
+
+__device__ static inline void xcd_walk(uint32_t nblk, uint32_t *vb0, uint32_t *vend, uint32_t *step)
+{
+    xcd_walk_at(blockIdx.x, gridDim.x, nblk, vb0, vend, step);
 }
 
 /* Stages one launch kind's range starts in LDS when they fit (the search
@@ -174,19 +179,59 @@ __device__ static inline __attribute__((always_inline)) void site_link_measured(
     __syncthreads(); /* su2_block_sum2's shared array is reused by the next block */
 }
 
+/* The end of a tick for lattice d by a workgroup of SU2_EN_BLOCK threads
+ * (the leading workgroups of a tick's last sweep launch): en_av pass 2 of
+ * its pending measurement, whose partials this tick's first update launch
+ * formed, with the sums and order of k_ben_av_final's 1024-thread pass
+ * (su2_en_final_small), then its sweep counter and pending flag.  The host
+ * ends a tick this way only when no lattice measures or reunitarises after
+ * the sweep (no end-of-tick en_av or normalise launch follows it), so the
+ * end-of-tick measurement of k_ben_av_final never arises here.  No sweep
+ * launch reads the counters or, after the first, the pending flags. */
+__device__ static inline void tick_end_small(const su2_bdesc &d, uint32_t *__restrict__ ctr,
+                                             uint32_t *__restrict__ pend, int carry_on)
+{
+    const uint32_t s = ctr[d.slot], p = pend[d.slot];
+    __syncthreads(); /* every wave has read them before lane 0 sets them */
+    if (p) {
+        const uint32_t m = (s - 1u - (uint32_t)d.n_therm) / (uint32_t)d.n_cor;
+        su2_en_final_small(d.partc, d.en_nb, 1.0 / (2.0 * (double)d.g.Vh), d.resE + m, d.resP + m);
+    }
+    const bool later = measure_now(d, s) && deferred(d, s, carry_on);
+    if (threadIdx.x == 0) {
+        ctr[d.slot] = s + 1u;
+        pend[d.slot] = later ? 1u : 0u;
+    }
+    __syncthreads(); /* su2_en_final_small's shared array is reused by the next call */
+}
+
 /* OR = 0: link update; ALGO is the algorithm class of the resident lattices
  * (all heat-bath, all Metropolis, or mixed), so the common homogeneous case
  * carries one update body.  OR = 1: over-relaxation pass number or_pass
  * (lattices with fewer over-relaxation sweeps per tick skip it). */
-template <int MU, int OR, int ALGO, typename LinkT, bool MEAS = false>
+/* END: the tick's last sweep launch also ends the tick: its first `lead`
+ * workgroups (a multiple of 8, so the others keep their XCD) run
+ * tick_end_small for the lattices j = wg, wg + lead, ...; the workgroups
+ * after them sweep as usual. */
+template <int MU, int OR, int ALGO, typename LinkT, bool MEAS = false, bool END = false>
 __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H, const su2_bdesc *__restrict__ D,
                                                 const uint32_t *__restrict__ B, int parity, int or_pass,
-                                                const uint32_t *__restrict__ pend)
+                                                const uint32_t *__restrict__ pend, uint32_t *__restrict__ ctr,
+                                                uint32_t *__restrict__ pend_end, uint32_t lead, int carry_on)
 {
     __shared__ uint32_t s_begin[SU2_BATCH_LDS_DESC];
     const uint32_t n = H->n_active;
+    uint32_t wg = blockIdx.x, G = gridDim.x;
+    if constexpr (END) {
+        if (wg < lead) {
+            for (uint32_t j = wg; j < n; j += lead) tick_end_small(D[j], ctr, pend_end, carry_on);
+            return;
+        }
+        wg -= lead;
+        G -= lead;
+    }
     uint32_t vb0, vend, step;
-    xcd_walk(H->sw_blocks, &vb0, &vend, &step);
+    xcd_walk_at(wg, G, H->sw_blocks, &vb0, &vend, &step);
     if (vb0 >= vend) return;
     const bool lds = stage_begins(B, n, s_begin);
     if constexpr (OR) {
@@ -354,45 +399,57 @@ __global__ __launch_bounds__(256) void k_btick_end(const su2_bhead *__restrict__
 
 template <int MU, int OR, int ALGO, typename LinkT>
 hipError_t launch_bsweep(const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, int parity, int or_pass,
-                         uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend, int measure)
+                         uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend, int measure,
+                         const su2_btick_end *end)
 {
     if constexpr (MU == SU2_DIR_Y && !OR) {
         if (measure) {
             hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT, true>), dim3(grid), dim3(block), 0, st, H, D, B, parity,
-                               or_pass, pend);
+                               or_pass, pend, nullptr, nullptr, 0u, 0);
+            return hipGetLastError();
+        }
+    }
+    if constexpr (MU == SU2_DIR_T) {
+        if (end) {
+            hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT, false, true>), dim3(grid + end->lead), dim3(block), 0,
+                               st, H, D, B, parity, or_pass, pend, end->ctr, end->pend, end->lead, end->carry_on);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((k_bsweep<MU, OR, ALGO, LinkT>), dim3(grid), dim3(block), 0, st, H, D, B, parity, or_pass,
-                       pend);
+                       pend, nullptr, nullptr, 0u, 0);
     return hipGetLastError();
 }
 
 template <int OR, int ALGO, typename LinkT>
 hipError_t launch_bsweep_mu(int mu, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B, int parity,
                             int or_pass, uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend,
-                            int measure)
+                            int measure, const su2_btick_end *end)
 {
     switch (mu) {
-    case 0: return launch_bsweep<0, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, measure);
-    case 1: return launch_bsweep<1, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, 0);
-    default: return launch_bsweep<2, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, 0);
+    case 0: return launch_bsweep<0, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, measure, nullptr);
+    case 1: return launch_bsweep<1, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, 0, nullptr);
+    default: return launch_bsweep<2, OR, ALGO, LinkT>(H, D, B, parity, or_pass, grid, block, st, pend, 0, end);
     }
 }
 
 template <typename LinkT>
 hipError_t launch_bsweep_t(int mu, int over_relax, int algo_class, const su2_bhead *H, const su2_bdesc *D,
                            const uint32_t *B, int parity, int or_pass, uint32_t grid, uint32_t block, hipStream_t st,
-                           const uint32_t *pend, int measure)
+                           const uint32_t *pend, int measure, const su2_btick_end *end)
 {
     if (over_relax)
-        return launch_bsweep_mu<1, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, or_pass, grid, block, st, pend, 0);
+        return launch_bsweep_mu<1, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, or_pass, grid, block, st, pend, 0,
+                                                          end);
     switch (algo_class) {
     case SU2_BALGO_HB:
-        return launch_bsweep_mu<0, SU2_BALGO_HB, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure);
+        return launch_bsweep_mu<0, SU2_BALGO_HB, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure, end);
     case SU2_BALGO_METRO:
-        return launch_bsweep_mu<0, SU2_BALGO_METRO, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure);
-    default: return launch_bsweep_mu<0, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure);
+        return launch_bsweep_mu<0, SU2_BALGO_METRO, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure,
+                                                          end);
+    default:
+        return launch_bsweep_mu<0, SU2_BALGO_MIXED, LinkT>(mu, H, D, B, parity, 0, grid, block, st, pend, measure,
+                                                          end);
     }
 }
 
@@ -400,16 +457,20 @@ hipError_t launch_bsweep_t(int mu, int over_relax, int algo_class, const su2_bhe
 
 hipError_t su2_launch_bsweep(int f32, int mu, int over_relax, int algo_class, const su2_bhead *H,
                              const su2_bdesc *D, const uint32_t *B, uint32_t max_desc, int parity, int or_pass,
-                             uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend, int measure)
+                             uint32_t grid, uint32_t block, hipStream_t st, const uint32_t *pend, int measure,
+                             const su2_btick_end *end)
 {
     (void)max_desc; /* the sweep range starts are the first array of B */
     if (measure && (mu != SU2_DIR_Y || parity != 0 || over_relax || block != SU2_EN_BLOCK || !pend))
         return hipErrorInvalidValue;
+    if (end && (mu != SU2_DIR_T || parity != 1 || block != SU2_EN_BLOCK || !end->ctr || !end->pend ||
+                end->lead == 0 || (end->lead & 7u)))
+        return hipErrorInvalidValue;
     if (f32)
         return launch_bsweep_t<su2_l32>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st, pend,
-                                        measure);
+                                        measure, end);
     return launch_bsweep_t<su2_l64>(mu, over_relax, algo_class, H, D, B, parity, or_pass, grid, block, st, pend,
-                                    measure);
+                                    measure, end);
 }
 
 hipError_t su2_launch_bnormalize(int f32, const su2_bhead *H, const su2_bdesc *D, const uint32_t *B,

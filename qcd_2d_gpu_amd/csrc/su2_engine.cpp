@@ -157,6 +157,11 @@ struct tick_plan {
      * en_av pass 2 ends the tick; carry_on: the sweep launches run
      * SU2_EN_BLOCK threads, so measurements may be deferred */
     bool pre = false, post = false, meas = false, carry = false, carry_pre = false, fin = false, carry_on = false;
+    /* end: no launch follows the sweeps (no end-of-tick reunitarisation or
+     * en_av), so the last sweep launch ends the tick in leading workgroups
+     * instead of a launch of its own (a function of carry_on, post and
+     * meas, which the key holds) */
+    bool end = false;
     int n_or = 0;
     int algo = SU2_BALGO_MIXED;
     uint32_t g_sw = 0, g_nm = 0, g_en = 0, g_fin = 0; /* workgroups per launch kind */
@@ -325,6 +330,7 @@ struct su2_engine {
         }
         p.carry = pend && p.carry_on;
         p.carry_pre = pend && !p.carry_on;
+        p.end = p.carry_on && !p.post && !p.meas;
         p.algo = hb && metro ? SU2_BALGO_MIXED : (hb ? SU2_BALGO_HB : SU2_BALGO_METRO);
         p.g_sw = fit_grid(hH->sw_blocks, (uint32_t)grid);
         p.g_nm = fit_grid(hH->nm_blocks, (uint32_t)aux_grid);
@@ -341,19 +347,28 @@ struct su2_engine {
         /* the pending measurements, of the field as the last tick left it */
         if (p.carry_pre) e = su2_launch_ben_av_partial(f32, dH, dD, dB, mj, dctr, dpend, p.g_en, p.carry_on, 1, st);
         if (p.pre && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 0, p.g_nm, st);
+        /* the tick's last sweep launch, (t, parity 1) of the last pass, ends
+         * the tick when nothing follows it (p.end) */
+        const su2_btick_end te = {dctr, dpend, (p.g_fin + 7u) & ~7u, p.carry_on};
         for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
-            for (int par = 0; par < 2 && e == hipSuccess; ++par)
+            for (int par = 0; par < 2 && e == hipSuccess; ++par) {
+                const bool last = p.end && p.n_or == 0 && mu == SU2_DIR_T && par == 1;
                 e = su2_launch_bsweep(f32, mu, 0, p.algo, dH, dD, dB, mj, par, 0, p.g_sw, block, st, dpend,
-                                      p.carry && mu == 0 && par == 0);
+                                      p.carry && mu == 0 && par == 0, last ? &te : nullptr);
+            }
         for (int pass = 0; pass < p.n_or && e == hipSuccess; ++pass)
             for (int mu = 0; mu < 3 && e == hipSuccess; ++mu)
-                for (int par = 0; par < 2 && e == hipSuccess; ++par)
-                    e = su2_launch_bsweep(f32, mu, 1, p.algo, dH, dD, dB, mj, par, pass, p.g_sw, block, st);
+                for (int par = 0; par < 2 && e == hipSuccess; ++par) {
+                    const bool last = p.end && pass == p.n_or - 1 && mu == SU2_DIR_T && par == 1;
+                    e = su2_launch_bsweep(f32, mu, 1, p.algo, dH, dD, dB, mj, par, pass, p.g_sw, block, st, nullptr,
+                                          0, last ? &te : nullptr);
+                }
         if (p.post && e == hipSuccess) e = su2_launch_bnormalize(f32, dH, dD, dB, mj, dctr, 1, p.g_nm, st);
         if (p.meas && e == hipSuccess)
             e = su2_launch_ben_av_partial(f32, dH, dD, dB, mj, dctr, dpend, p.g_en, p.carry_on, 0, st);
-        if (p.fin && e == hipSuccess) e = su2_launch_ben_av_final(dH, dD, dctr, dpend, p.g_fin, p.carry_on, st);
-        if (!p.fin && e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
+        if (!p.end && p.fin && e == hipSuccess)
+            e = su2_launch_ben_av_final(dH, dD, dctr, dpend, p.g_fin, p.carry_on, st);
+        if (!p.end && !p.fin && e == hipSuccess) e = su2_launch_btick_end(dH, dD, dctr, st);
         if (e != hipSuccess) return su2_set_error(SU2_EHIP, "tick launch: %s", hipGetErrorString(e));
         return SU2_OK;
     }
