@@ -127,10 +127,6 @@ __device__ static inline bool normalize_post(const su2_bdesc &d, uint32_t s)
     return (i % (uint32_t)d.nevery) == 0u;
 }
 
-/* One link update of lattice d at half-site h.  The heat-bath runs with the
- * workgroup-shared rejection loop, so every thread of the workgroup calls
- * this for each virtual block (in_range = 0 past the end of the lattice);
- * the algorithm is uniform over a virtual block. */
 /* Acceptance of heat-bath and over-relaxation, which accept every update,
  * is counted once per sweep, in the t launches (SU2_K_ACC_SWEEP); Metropolis
  * counts in every launch. */
@@ -140,16 +136,22 @@ __device__ static inline constexpr int accept_once_per_sweep()
     return MU == SU2_DIR_T ? SU2_K_ACC_SWEEP : SU2_K_ACC_NONE;
 }
 
+/* One link update of lattice d at half-site h (in_range = 0 past the end of
+ * the lattice); the algorithm is uniform over a virtual block.  The
+ * heat-bath runs its Kennedy-Pendleton loop in each lane, as the
+ * single-lattice heat-bath of a cache-resident lattice does: measured
+ * against the workgroup-shared rejection loop, alternating on one box, the
+ * 64-lattice ensemble -5 %, a lone job's tick -8 to -11 % at 32^3-64^3,
+ * the served p50 -10 %; the results are the same bits. */
 template <int MU, int ALGO, typename LinkT>
 __device__ static inline __attribute__((always_inline)) void site_link(const su2_bdesc &d, int parity, uint32_t h,
-                                                                       int in_range, su2_kp_queue &q)
+                                                                       int in_range)
 {
+    if (!in_range) return;
     if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
-        /* early RNG load for FP32 links only (measured: batched FP32 -4.5 %,
-         * FP64 +1.6 % per ensemble run; see su2_sweep_site_hb_wg) */
-        su2_sweep_site_hb_wg<MU, SU2_K_BC_ANY, LinkT, sizeof(LinkT) == sizeof(su2_l32)>(
-            (LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, in_range, q, d.bc, accept_once_per_sweep<MU>());
-    else if (in_range)
+        su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_HB, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, d.bc,
+                                                         accept_once_per_sweep<MU>());
+    else
         su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_METRO, LinkT>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h,
                                                             d.bc);
 }
@@ -159,16 +161,17 @@ __device__ static inline __attribute__((always_inline)) void site_link(const su2
  * goes to its carried partials, the bits su2_en_av_owner_block writes. */
 template <int MU, int ALGO, typename LinkT>
 __device__ static inline __attribute__((always_inline)) void site_link_measured(const su2_bdesc &d, int parity,
-                                                                                uint32_t h, int in_range,
-                                                                                su2_kp_queue &q, uint32_t lb)
+                                                                                uint32_t h, int in_range, uint32_t lb)
 {
     su2_owner_plaq m = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
-    if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
-        su2_sweep_site_hb_wg<MU, SU2_K_BC_ANY, LinkT, sizeof(LinkT) == sizeof(su2_l32), false, true>(
-            (LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity, h, in_range, q, d.bc, accept_once_per_sweep<MU>(), &m);
-    else if (in_range)
-        su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_METRO, LinkT, true>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity,
-                                                                  h, d.bc, SU2_K_ACC_LAUNCH, &m);
+    if (in_range) {
+        if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
+            su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_HB, LinkT, true>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity,
+                                                                   h, d.bc, accept_once_per_sweep<MU>(), &m);
+        else
+            su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_METRO, LinkT, true>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph,
+                                                                      parity, h, d.bc, SU2_K_ACC_LAUNCH, &m);
+    }
     double e = 0.0, pl = 0.0;
     if (in_range) su2_owner_finish(m, d.bc == SU2_K_BC_PMC, &e, &pl);
     su2_block_sum2(e, pl, SU2_EN_BLOCK);
@@ -247,7 +250,6 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
                                                              d.bc, accept_once_per_sweep<MU>());
         }
     } else {
-        __shared__ su2_kp_queue q;
         for (uint32_t vb = vb0; vb < vend; vb += step) {
             /* everything up to the update is uniform in the workgroup */
             const su2_bdesc &d = D[find_desc(B, s_begin, lds, n, vb)];
@@ -256,11 +258,11 @@ __global__ __launch_bounds__(256) void k_bsweep(const su2_bhead *__restrict__ H,
             const int in_range = h < d.g.Vh;
             if constexpr (MEAS) {
                 if (pend[d.slot]) {
-                    site_link_measured<MU, ALGO, LinkT>(d, parity, h, in_range, q, lb);
+                    site_link_measured<MU, ALGO, LinkT>(d, parity, h, in_range, lb);
                     continue;
                 }
             }
-            site_link<MU, ALGO, LinkT>(d, parity, h, in_range, q);
+            site_link<MU, ALGO, LinkT>(d, parity, h, in_range);
         }
     }
 }

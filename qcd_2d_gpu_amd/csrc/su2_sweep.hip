@@ -54,7 +54,7 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
     const uint32_t h = lb * nthr + threadIdx.x;
     if constexpr (MEAS) {
         su2_owner_plaq m = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
-        if constexpr (UPD == SU2_K_HB && sizeof(LinkT) == sizeof(su2_l64)) {
+        if constexpr (UPD == SU2_K_HB && STREAM_RNG) {
             __shared__ su2_kp_queue q;
             su2_sweep_site_hb_wg<MU, BC, LinkT, true, STREAM_RNG, true>(U, rng, accept, g, ph, parity, h, h < g.Vh,
                                                                        q, BC, acc_mode, &m);
@@ -70,11 +70,16 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
         }
         return;
     }
-    /* Heat-bath with FP64 links shares the rejection loop across the
-     * workgroup (su2_sweep_site_hb_wg, early RNG load); with FP32 links each
-     * lane runs its own loop (su2_sweep_site), measured 3 % faster per
-     * launch at 128^3 (28.7 against 29.6 us) with FP64 links unchanged. */
-    if constexpr (UPD == SU2_K_HB && sizeof(LinkT) == sizeof(su2_l64)) {
+    /* The heat-bath runs its Kennedy-Pendleton loop in each lane
+     * (su2_sweep_site), except the streaming heat-bath of a lattice beyond
+     * the Infinity Cache, which shares the rejection loop across the
+     * workgroup (su2_sweep_site_hb_wg).  Measured with FP64 links,
+     * alternating on one box, per-lane against shared: 16^3 and 32^3 -10 %,
+     * 64^3 -5 %, 128^3 -1 %, 256^3 +1 %; FP32 links: 128^3 -3 % (28.7
+     * against 29.6 us).  The wavefront repeats its trial for its slowest lane
+     * (1.8 trials per wavefront against 1.02 per site at 128^3, beta = 5),
+     * but saves the queue's three barriers and LDS round trips. */
+    if constexpr (UPD == SU2_K_HB && STREAM_RNG) {
         __shared__ su2_kp_queue q;
         su2_sweep_site_hb_wg<MU, BC, LinkT, true, STREAM_RNG>(U, rng, accept, g, ph, parity, h, h < g.Vh, q, BC,
                                                              acc_mode);
