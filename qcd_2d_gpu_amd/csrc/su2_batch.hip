@@ -58,9 +58,13 @@ __device__ static inline void xcd_walk(uint32_t nblk, uint32_t *vb0, uint32_t *v
 }
 
 /* Stages one launch kind's range starts in LDS when they fit (the search
- * then costs no dependent global loads); returns whether they did. */
+ * then costs no dependent global loads); returns whether they did.  A lone
+ * lattice needs no search (find_desc returns 0 without a load), so nothing
+ * is staged for it and the workgroup skips the load and the barrier (n is
+ * uniform over the workgroup). */
 __device__ static inline bool stage_begins(const uint32_t *__restrict__ Bk, uint32_t n, uint32_t *s_begin)
 {
+    if (n <= 1u) return false;
     const bool lds = n <= SU2_BATCH_LDS_DESC;
     if (lds)
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_begin[i] = Bk[i];
