@@ -357,10 +357,12 @@ def lattice_launch_bench(Lt, Lx, Ly, reps=5, therm=2):
 def latency_floor_bench():
     """What a phase of a small lattice's sweep costs: su2_phase_probe (one
     phase = every thread writes 8 doubles of its own workgroup's, then reads
-    8 of the next workgroup's) separated by back-to-back kernel launches or
-    by a grid-wide barrier in one cooperative launch, for 8, 32 and 128
-    workgroups of 256; beside it the heat-bath and over-relaxation launches
-    of 16^3 and 32^3 lattices (2048 and 16384 updates per launch)."""
+    8 of the next workgroup's) separated by back-to-back kernel launches, by
+    a grid-wide barrier in one cooperative launch, or by a barrier among the
+    workgroups of one XCD only (one group per XCD, side by side), for 8, 32
+    and 128 workgroups of 256 (1, 4 and 16 per XCD); beside it the heat-bath
+    and over-relaxation launches of 16^3 and 32^3 lattices (2048 and 16384
+    updates per launch)."""
     import qcd_2d_gpu_amd as q
     from qcd_2d_gpu_amd import _lib as L
 
@@ -372,10 +374,12 @@ def latency_floor_bench():
     for blocks in (8, 32, 128):
         buf = q.DeviceBuffer(blocks * 256 * words * 8)
         buf.zero(stream)
-        lu, bu = C.c_double(), C.c_double()
-        L.check(lib.su2_phase_probe(buf.ptr, blocks, words, 200, stream, C.byref(lu), C.byref(bu)), "phase_probe")
+        lu, bu, xu = C.c_double(), C.c_double(), C.c_double()
+        L.check(lib.su2_phase_probe(buf.ptr, blocks, words, 200, stream, C.byref(lu), C.byref(bu), C.byref(xu)),
+                "phase_probe")
         buf.free()
-        out["phase_separation"].append({"workgroups": blocks, "launch_us": lu.value, "grid_barrier_us": bu.value})
+        out["phase_separation"].append({"workgroups": blocks, "launch_us": lu.value, "grid_barrier_us": bu.value,
+                                        "xcd_barrier_us": xu.value})
     lib.su2_stream_destroy(stream)
     for n in (16, 32):
         r = lattice_launch_bench(n, n, n, reps=20, therm=20)

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define SU2MC_ABI_VERSION 4
+#define SU2MC_ABI_VERSION 5
 
 /* ---- status codes ---- */
 #define SU2_OK 0
@@ -239,13 +239,18 @@ int su2_stream_probe(void *dst, const void *src, size_t bytes, int reads, int un
  * of `blocks` workgroups of 256 adding 1 to `words` doubles of its own and
  * then reading `words` doubles of the next workgroup, as a (direction,
  * parity) launch depends on the one before it.  The halves are separated
- * once by kernel boundaries (back-to-back launches on `stream`) and once by
- * a grid-wide barrier inside one cooperative launch; *launch_us and
- * *barrier_us are the mean time per separation over `phases` phases.  buf
- * holds at least blocks * 256 * words doubles; blocks must all fit on the
- * device at once.  Synchronous. */
+ * once by kernel boundaries (back-to-back launches on `stream`), once by a
+ * grid-wide barrier inside one cooperative launch, and, unless
+ * xcd_barrier_us is NULL, once by a barrier among only the workgroups on one
+ * XCD (each finds its XCD from the hardware's XCC_ID register and pairs
+ * with the next workgroup of its own XCD; the groups run side by side).
+ * *launch_us, *barrier_us and *xcd_barrier_us are the mean time per
+ * separation over `phases` phases.  buf holds at least blocks * 256 * words
+ * doubles; blocks must all fit on the device at once.  SU2_EHIP when the
+ * XCD-local barrier read stale data or one of its bounded waits ran out.
+ * Synchronous. */
 int su2_phase_probe(double *buf, int blocks, int words, int phases, void *stream, double *launch_us,
-                    double *barrier_us);
+                    double *barrier_us, double *xcd_barrier_us);
 int su2_stream_create(void **stream);
 int su2_stream_destroy(void *stream);
 int su2_stream_synchronize(void *stream);

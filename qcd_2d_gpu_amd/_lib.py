@@ -116,7 +116,8 @@ PROTOTYPES = {
     "su2_memset_zero": (_I, [_P, C.c_size_t, _P]),
     "su2_device_copy": (_I, [_P, _P, C.c_size_t, _P]),
     "su2_stream_probe": (_I, [_P, _P, C.c_size_t, _I, _I, _I, _I, _P]),
-    "su2_phase_probe": (_I, [_P, _I, _I, _I, _P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "su2_phase_probe": (_I, [_P, _I, _I, _I, _P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                         C.POINTER(C.c_double)]),
     "su2_stream_create": (_I, [C.POINTER(C.c_void_p)]),
     "su2_stream_destroy": (_I, [_P]),
     "su2_stream_synchronize": (_I, [_P]),
@@ -177,7 +178,7 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.su2_abi_version() != 4:
+    if lib.su2_abi_version() != 5:
         raise RuntimeError("libsu2mc ABI version mismatch")
     _lib = lib
     return lib

@@ -474,12 +474,15 @@ int su2_stream_probe(void *dst, const void *src, size_t bytes, int reads, int un
 }
 
 int su2_phase_probe(double *buf, int blocks, int words, int phases, void *stream, double *launch_us,
-                    double *barrier_us)
+                    double *barrier_us, double *xcd_barrier_us)
 {
     if (!buf || !launch_us || !barrier_us || blocks < 1 || blocks > 1024 || words < 1 || words > 64 || phases < 1 ||
         phases > 100000)
         return fail(SU2_EINVAL, "phase_probe: buf, outputs, 1 <= blocks <= 1024, 1 <= words <= 64, 1 <= phases <= 100000");
-    hipError_t e = su2_phase_probe_run(buf, (uint32_t)blocks, words, phases, S(stream), launch_us, barrier_us);
+    hipError_t e =
+        su2_phase_probe_run(buf, (uint32_t)blocks, words, phases, S(stream), launch_us, barrier_us, xcd_barrier_us);
+    if (e == hipErrorUnknown)
+        return fail(SU2_EHIP, "phase probe: an XCD-local barrier ran out of spins or a read after it was stale");
     return e == hipSuccess ? SU2_OK : hip_fail(e, "phase probe");
 }
 

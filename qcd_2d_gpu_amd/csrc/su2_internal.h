@@ -183,9 +183,10 @@ hipError_t su2_launch_fill_i32(int *p, int value, size_t n, hipStream_t st);
 hipError_t su2_launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 hipError_t su2_launch_stream_probe(void *dst, const void *src, size_t bytes, int reads, int unroll, int nt,
                                    uint32_t blocks, hipStream_t st);
-/* su2_phase_probe: synchronous; launches on st and waits for it */
+/* su2_phase_probe: synchronous; launches on st and waits for it;
+ * hipErrorUnknown when an XCD-local barrier ran out of spins or read stale data */
 hipError_t su2_phase_probe_run(double *buf, uint32_t blocks, int words, int phases, hipStream_t st, double *launch_us,
-                               double *barrier_us);
+                               double *barrier_us, double *xcd_barrier_us);
 
 /* su2_set_error (defined in su2_capi.cpp), shared with the C host driver */
 #include "su2_host.h"

@@ -83,7 +83,7 @@ def test_invalid_arguments_rejected_before_launch():
     assert "V = 7" in lib.su2_last_error().decode()
     lu, bu = C.c_double(), C.c_double()
     for args in ((None, 8, 8, 10), (fake, 0, 8, 10), (fake, 2000, 8, 10), (fake, 8, 0, 10), (fake, 8, 8, 0)):
-        assert lib.su2_phase_probe(*args, None, C.byref(lu), C.byref(bu)) == L.SU2_EINVAL
+        assert lib.su2_phase_probe(*args, None, C.byref(lu), C.byref(bu), None) == L.SU2_EINVAL
     assert lib.su2_set_sweep_block(100) == L.SU2_EINVAL
     assert lib.su2_set_sweep_block(0) == L.SU2_OK
     assert lib.su2_strerror(L.SU2_EODD_EXTENT) == b"lattice extents must be even"
