@@ -265,7 +265,7 @@ def _res_slots(q, n):
 def test_measured_sweep_equals_en_av_then_sweep(gpu, oracle_mod, prec, shape, bc, algo, update):
     """su2_mc_up_measured (the measurement carried by the sweep's first
     launch) against su2_en_av followed by the same sweep: links, RNG states,
-    acceptance counters and both measured values bit for bit, three rounds in
+    acceptance counters and both measured values bit for bit, three passes in
     a row; the measurement also against the checker's en_av to 1e-12, and
     with a sweep block of 128 threads (the measurement then runs on its own)
     the same bits again."""
@@ -293,12 +293,12 @@ def test_measured_sweep_equals_en_av_then_sweep(gpu, oracle_mod, prec, shape, bc
                 lib.su2_set_sweep_block(0)
             pa = ga.field_ref()
             for g, name in ((gb, "measured"), (gc, "measured, block 128")):
-                assert _bits_equal(g.field_ref(), pa), f"round {r} {name}: links differ"
-                assert np.array_equal(g.rng_state(), ga.rng_state()), f"round {r} {name}: RNG differs"
-                assert np.array_equal(g.accept(), ga.accept()), f"round {r} {name}: acceptance differs"
+                assert _bits_equal(g.field_ref(), pa), f"pass {r} {name}: links differ"
+                assert np.array_equal(g.rng_state(), ga.rng_state()), f"pass {r} {name}: RNG differs"
+                assert np.array_equal(g.accept(), ga.accept()), f"pass {r} {name}: acceptance differs"
         ra, rb, rc = (s.download(np.float64, 6).reshape(3, 2) for s in slots)
         assert _bits_equal(rb, ra) and _bits_equal(rc, ra), (ra, rb, rc)
-        # the measurement of round 0 is of the start field
+        # the measurement of pass 0 is of the start field
         g0 = q.Lattice(Lt, Lx, Ly, 3.0, 0.4, 2, precision=prec)
         try:
             g0.setup_kernel(77)
