@@ -167,17 +167,15 @@ template <int MU, int ALGO, typename LinkT>
 __device__ static inline __attribute__((always_inline)) void site_link_measured(const su2_bdesc &d, int parity,
                                                                                 uint32_t h, int in_range, uint32_t lb)
 {
-    su2_owner_plaq m = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+    double e = 0.0, pl = 0.0;
     if (in_range) {
         if (ALGO == SU2_BALGO_HB || (ALGO == SU2_BALGO_MIXED && d.algo == 1))
             su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_HB, LinkT, true>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph, parity,
-                                                                   h, d.bc, accept_once_per_sweep<MU>(), &m);
+                                                                   h, d.bc, accept_once_per_sweep<MU>(), &e, &pl);
         else
             su2_sweep_site<MU, SU2_K_BC_ANY, SU2_K_METRO, LinkT, true>((LinkT *)d.U, d.rng, d.accept, d.g, d.ph,
-                                                                      parity, h, d.bc, SU2_K_ACC_LAUNCH, &m);
+                                                                      parity, h, d.bc, SU2_K_ACC_LAUNCH, &e, &pl);
     }
-    double e = 0.0, pl = 0.0;
-    if (in_range) su2_owner_finish(m, d.bc == SU2_K_BC_PMC, &e, &pl);
     su2_block_sum2(e, pl, SU2_EN_BLOCK);
     if (threadIdx.x == 0) {
         d.partc[2 * lb] = e;

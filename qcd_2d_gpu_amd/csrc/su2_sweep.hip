@@ -53,16 +53,17 @@ __device__ static inline __attribute__((always_inline)) void sweep_body(LinkT *_
     const uint32_t lb = su2_block_at(wg, nwg, order, strip, bpp);
     const uint32_t h = lb * nthr + threadIdx.x;
     if constexpr (MEAS) {
-        su2_owner_plaq m = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+        double e = 0.0, pl = 0.0;
         if constexpr (UPD == SU2_K_HB && STREAM_RNG) {
             __shared__ su2_kp_queue q;
+            su2_owner_plaq m = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
             su2_sweep_site_hb_wg<MU, BC, LinkT, true, STREAM_RNG, true>(U, rng, accept, g, ph, parity, h, h < g.Vh,
                                                                        q, BC, acc_mode, &m);
+            if (h < g.Vh) su2_owner_finish(m, BC == SU2_K_BC_PMC, &e, &pl);
         } else {
-            if (h < g.Vh) su2_sweep_site<MU, BC, UPD, LinkT, true>(U, rng, accept, g, ph, parity, h, BC, acc_mode, &m);
+            if (h < g.Vh)
+                su2_sweep_site<MU, BC, UPD, LinkT, true>(U, rng, accept, g, ph, parity, h, BC, acc_mode, &e, &pl);
         }
-        double e = 0.0, pl = 0.0;
-        if (h < g.Vh) su2_owner_finish(m, BC == SU2_K_BC_PMC, &e, &pl);
         su2_block_sum2(e, pl, SU2_EN_BLOCK);
         if (threadIdx.x == 0) {
             meas_work[2 * lb] = e;
