@@ -378,7 +378,8 @@ def latency_floor_bench():
         L.check(lib.su2_phase_probe(buf.ptr, blocks, words, 200, stream, C.byref(lu), C.byref(bu), C.byref(xu)),
                 "phase_probe")
         buf.free()
-        out["phase_separation"].append({"workgroups": blocks, "launch_us": lu.value, "grid_barrier_us": bu.value,
+        out["phase_separation"].append({"workgroups": blocks, "workgroups_per_xcd": blocks // 8,
+                                        "launch_us": lu.value, "grid_barrier_us": bu.value,
                                         "xcd_barrier_us": xu.value})
     lib.su2_stream_destroy(stream)
     for n in (16, 32):
